@@ -1,0 +1,124 @@
+/*
+ * lfts_gpu.h — C ABI of the MI355X (gfx950) L-FTS Langevin-step library.
+ *
+ * The reference (tmbeardsley/lfts_cpu) has no FFI: its boundary is the in-process C++
+ * class API that src/fts_cpu.cc drives. Each entry point below replaces one method of
+ * that API (cited). Conventions:
+ *   - one opaque context per system (replica); calls on one context are stream-ordered
+ *     on the context's own HIP stream and are not thread-safe per context;
+ *   - every call returns 0 on success and a non-zero LFTS_E* code on failure; the text
+ *     of the last failure is available from lfts_last_error(ctx);
+ *   - host buffers are caller-owned and never retained after a call returns;
+ *   - "dev" entry points take device pointers (caller-owned HBM), for callers that keep
+ *     their inputs resident on the GPU;
+ *   - fields are FP64, in the reference's order [W- | w+ | phi- | phi+], each of length
+ *     M = m[0]*m[1]*m[2], row-major p = (i*m[1] + j)*m[2] + k (src/field_generator.h:18).
+ */
+#ifndef LFTS_GPU_H
+#define LFTS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LFTS_OK = 0,
+    LFTS_EINVAL = 1,   /* bad argument or unsupported mesh size */
+    LFTS_EHIP = 2,     /* HIP runtime error (device missing, out of memory, launch failure) */
+    LFTS_ESTATE = 3,   /* call out of order (e.g. S(k) read before any sample) */
+    LFTS_ENUM = 4      /* non-finite value detected (NaN/Inf guard) */
+};
+
+typedef struct lfts_desc {
+    int N;        /* chain length (src/lfts_params.h:25) */
+    int NA;       /* A-block length; NB = N - NA */
+    int m[3];     /* mesh points mx, my, mz */
+    double L[3];  /* box edges in units of R0 */
+    int nhMax;    /* Anderson history depth; the reference uses 10 (src/lfts_simulation.cc:88) */
+} lfts_desc;
+
+typedef struct lfts_ctx lfts_ctx;
+
+/* Replaces the constructors diblock(NA,NB,m,L,M,Mk) (src/diblock.cc:9-25),
+ * step(...) with its g(k) table (src/step.cc:10-28,69-89), anderson(M,10)
+ * (src/anderson.cc:8-21) and the device state of langevin/strFunc. */
+int lfts_create(const lfts_desc* desc, lfts_ctx** out);
+int lfts_destroy(lfts_ctx* ctx);
+const char* lfts_last_error(const lfts_ctx* ctx);
+
+/* The context's HIP stream (hipStream_t), for callers that time or order work on it. */
+int lfts_get_stream(lfts_ctx* ctx, void** stream);
+
+/* Host <-> device copies of the field array w_ (src/lfts_simulation.h:21):
+ * nfields = 2 copies [W-|w+], nfields = 4 copies [W-|w+|phi-|phi+]. */
+int lfts_set_fields(lfts_ctx* ctx, const double* w, int nfields);
+int lfts_get_fields(lfts_ctx* ctx, double* w, int nfields);
+/* Device pointer to the context's resident field array (4*M doubles). */
+int lfts_fields_dev(lfts_ctx* ctx, double** w_dev);
+
+/* diblock::calc_concs(double* w) (src/diblock.cc:30-60): one propagator evaluation
+ * from [W-|w+]; writes [phi-|phi+] and returns ln Q. */
+int lfts_calc_concs(lfts_ctx* ctx, double* lnQ);
+
+/* anderson::mix(diblock*, maxIter, errTol, w) (src/anderson.cc:24-93): saddle-point
+ * solve for w+. *k_ret receives mix's return value (executed iterations + 1); *err_ret
+ * (may be NULL) the last residual. The small LU solve runs on the host. */
+int lfts_anderson_mix(lfts_ctx* ctx, int maxIter, double errTol, int* k_ret, double* err_ret);
+
+/* langevin(RNG, sigma, M) constructor's first draw (src/langevin.cc:7-19): sets the
+ * "previous" noise eta_prev (M host doubles). */
+int lfts_langevin_init(lfts_ctx* ctx, const double* eta_prev);
+/* langevin::step_wm(w, RNG, XbN, sigma, dt) (src/langevin.cc:23-37) with the new noise
+ * eta_new (M doubles) drawn by the caller: host pointer / device pointer variants. */
+int lfts_langevin_step(lfts_ctx* ctx, const double* eta_new, double XbN, double dt);
+int lfts_langevin_step_dev(lfts_ctx* ctx, const double* eta_new_dev, double XbN, double dt);
+
+/* One whole Langevin step of lfts_simulation::equilibrate()/statistics()
+ * (src/lfts_simulation.cc:67-87): step_wm with device noise, mix(maxIter, errTol), then
+ * the extra calc_concs that prints lnQ. Returns lnQ and the mix return value. */
+int lfts_langevin_full_step_dev(lfts_ctx* ctx, const double* eta_new_dev, double XbN, double dt,
+                                int maxIter, double errTol, double* lnQ, int* k_ret);
+
+/* strFunc::sample(w) (src/strFunc.cc:36-46): S_k += |FFT(W-)_k|^2 on the half spectrum
+ * k = k2 + (m2/2+1)(K1 + m1*K0). lfts_sk_get copies S (Mk doubles) and the count. */
+int lfts_sk_sample(lfts_ctx* ctx);
+int lfts_sk_get(lfts_ctx* ctx, double* S, int* nsamples);
+int lfts_sk_reset(lfts_ctx* ctx);
+
+/* strFunc::calcK + the stable sort of the constructor (src/strFunc.cc:27-31,80-103),
+ * host-side: K (Mk doubles), wt (Mk ints), P (Mk ints). No context needed. */
+int lfts_sk_bins(const int* m, const double* L, double* K, int* wt, int* P);
+/* strFunc::save(fileName, dp) (src/strFunc.cc:50-76) from a host S array. */
+int lfts_sk_save(const char* fileName, const int* m, const double* L, const double* S, int nsamples,
+                 double CV, double chi_b, double dK, int dp);
+
+/* lfts_simulation::getH() (src/lfts_simulation.cc:122-135): calc_concs, then
+ * H = -lnQ + (sum W-^2 / XbN - sum w+) / M. */
+int lfts_hamiltonian(lfts_ctx* ctx, double XbN, double* H);
+
+/* Testing access: stored propagator pair i (1..N) as the reference lays it out,
+ * [q_i | q+_{N+1-i}] (src/diblock.cc:19-24), 2*M host doubles. */
+int lfts_get_pair(lfts_ctx* ctx, int i, double* out);
+/* step::fwd(q_in, q_out, h, i) (src/step.cc:35-65) on host arrays (2*M each). */
+int lfts_step_fwd(lfts_ctx* ctx, const double* q_in, double* q_out, const double* h, int i);
+
+/* Counters: propagator evaluations and contour steps run on this context so far. */
+int lfts_counters(lfts_ctx* ctx, long long* n_prop_evals, long long* n_contour_steps);
+
+/* Host random stream of the reference: std::mt19937_64 seeded by the caller (the
+ * reference seeds 123456789, src/lfts_simulation.cc:77) feeding one persistent
+ * std::normal_distribution<double>(0, sigma) (src/langevin.cc:13,16,28); uniforms use a
+ * fresh std::uniform_real_distribution<double>(0,1) per call (src/lfts_simulation.cc:157). */
+typedef struct lfts_rng lfts_rng;
+int lfts_rng_create(unsigned long long seed, double sigma, lfts_rng** out);
+int lfts_rng_normals(lfts_rng* rng, double* out, long long n);
+int lfts_rng_uniforms(lfts_rng* rng, double* out, long long n);
+int lfts_rng_destroy(lfts_rng* rng);
+
+/* lfts_params::z_inf_discrete (src/lfts_params.cc:122-142). */
+double lfts_z_inf(const double* L, const int* m, int N, double Nbar, int tmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LFTS_GPU_H */

@@ -1,0 +1,770 @@
+// =====================================================================================
+// ORACLE — test infrastructure only.
+//
+// A CPU restatement of the arithmetic of tmbeardsley/lfts_cpu's Langevin step, written
+// from the reference's behaviour (cited file:line below), used as the parity checker for
+// the HIP path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load this library, and only as the checker or as the timed CPU baseline. The product
+// (lfts_cpu_amd/) never links, loads or calls it.
+//
+// The reference itself cannot be built here: it needs FFTW3 and GSL, which are absent, so
+// there is no oracle/_ref build. In their place this file carries:
+//   * its own mixed-radix complex FFT (Stockham autosort, radices 4/2/3/5/any prime),
+//     standing in for FFTW's r2c/c2r plans (src/step.cc:15-16, src/strFunc.cc:15);
+//   * its own LU with partial pivoting, standing in for GSL's gsl_linalg_LU_decomp /
+//     gsl_linalg_LU_solve (src/anderson.cc:118-126).
+// The batched-2 real transforms of a contour step are evaluated with the two-for-one
+// identity: z = q + i*q_dagger is transformed once, multiplied by the real, even kernel
+// g(k) and transformed back; Re/Im are then exactly the two real convolutions that
+// r2c -> *g -> c2r gives (src/step.cc:35-65). Reductions keep the reference's serial
+// summation order.
+//
+// Pinning (what checks each function; tests/test_oracle_*.py):
+//   fft            brute-force O(n^2) DFT, numpy.fft, Parseval, plane waves
+//   g table        closed form on tiny meshes, Nyquist convention
+//   calc_concs     closed forms (w=0; uniform w+; uniform W-), the invariant that
+//                  (1/M) sum_r q_i q+_i / h is the same for every bead i, mean(phi+)=1,
+//                  an independent numpy restatement that uses numpy.fft.rfftn/irfftn
+//   anderson       converged residual below tolerance, fixed point reproduced,
+//                  independent numpy restatement of the iterate sequence, LU vs scipy
+//   langevin       closed-form update; the noise stream against the C++ standard's
+//                  mt19937_64 known answer and a Python restatement of the polar method
+//   strFunc        single cosine mode (|w_k|^2 = (A M/2)^2), bins vs numpy, weights
+//   getH           closed form for uniform fields
+//   z_inf          mpmath at 50 digits
+// =====================================================================================
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+typedef std::complex<double> cplx;
+
+// -------------------------------------------------------------------------------------
+// FFT: unnormalised complex DFT, X[k] = sum_n x[n] exp(-2 pi i n k / n), any length.
+// Stockham autosort: after the stage that completes length-L sub-DFTs, y[j*(n/L)+k]
+// holds the length-L DFT (index j) of the stride-(n/L) subsequence starting at k.
+// -------------------------------------------------------------------------------------
+struct fft_plan1d {
+    int n = 1;
+    std::vector<int> radix;
+    std::vector<cplx> w;  // w[k] = exp(-2 pi i k / n)
+};
+
+static void make_plan(fft_plan1d& P, int n) {
+    P.n = n;
+    P.radix.clear();
+    int r = n;
+    while (r % 4 == 0) { P.radix.push_back(4); r /= 4; }
+    while (r % 2 == 0) { P.radix.push_back(2); r /= 2; }
+    for (int p = 3; p * p <= r; p += 2)
+        while (r % p == 0) { P.radix.push_back(p); r /= p; }
+    if (r > 1) P.radix.push_back(r);
+    P.w.resize(n);
+    for (int k = 0; k < n; k++) {
+        double a = -2.0 * M_PI * (double)k / (double)n;
+        P.w[k] = cplx(std::cos(a), std::sin(a));
+    }
+}
+
+// forward transform of x (length n) in place, using scratch y (length n)
+static void fft_forward(const fft_plan1d& P, cplx* x, cplx* y) {
+    const int n = P.n;
+    int L = 1;
+    cplx* src = x;
+    cplx* dst = y;
+    cplx a[64], b[64];
+    std::vector<cplx> abig, bbig;
+    for (int p : P.radix) {
+        cplx* A = a;
+        cplx* B = b;
+        if (p > 64) { abig.resize(p); bbig.resize(p); A = abig.data(); B = bbig.data(); }
+        const int mp = n / (L * p);
+        for (int j = 0; j < L; j++) {
+            for (int k = 0; k < mp; k++) {
+                for (int q = 0; q < p; q++) {
+                    cplx v = src[j * p * mp + k + q * mp];
+                    long tw = ((long)q * j * mp) % n;
+                    A[q] = (tw == 0) ? v : v * P.w[tw];
+                }
+                for (int s = 0; s < p; s++) {
+                    cplx acc = A[0];
+                    for (int q = 1; q < p; q++) {
+                        long e = ((long)q * s) % p;
+                        acc += (e == 0) ? A[q] : A[q] * P.w[e * (n / p)];
+                    }
+                    B[s] = acc;
+                }
+                for (int s = 0; s < p; s++) dst[(j + s * L) * mp + k] = B[s];
+            }
+        }
+        std::swap(src, dst);
+        L *= p;
+    }
+    if (src != x) std::memcpy(x, src, sizeof(cplx) * n);
+}
+
+static void fft_1d(const fft_plan1d& P, cplx* x, cplx* y, int sign) {
+    if (sign < 0) { fft_forward(P, x, y); return; }
+    for (int i = 0; i < P.n; i++) x[i] = std::conj(x[i]);
+    fft_forward(P, x, y);
+    for (int i = 0; i < P.n; i++) x[i] = std::conj(x[i]);
+}
+
+// 3D transform of a row-major m[0] x m[1] x m[2] complex array (last index fastest)
+static void fft_3d(const fft_plan1d* P, const int* m, cplx* a, int sign) {
+    const long M = (long)m[0] * m[1] * m[2];
+    for (int ax = 2; ax >= 0; ax--) {
+        const int n = m[ax];
+        if (n == 1) continue;
+        long stride = 1;
+        for (int d = ax + 1; d < 3; d++) stride *= m[d];
+        const long nlines = M / n;
+        #pragma omp parallel
+        {
+            std::vector<cplx> buf(n), scr(n);
+            #pragma omp for schedule(static)
+            for (long l = 0; l < nlines; l++) {
+                long outer = l / stride, inner = l % stride;
+                long base = outer * stride * n + inner;
+                for (int e = 0; e < n; e++) buf[e] = a[base + e * stride];
+                fft_1d(P[ax], buf.data(), scr.data(), sign);
+                for (int e = 0; e < n; e++) a[base + e * stride] = buf[e];
+            }
+        }
+    }
+}
+
+// -------------------------------------------------------------------------------------
+// step: g table and one contour step (src/step.cc:10-89)
+// -------------------------------------------------------------------------------------
+struct orc_step {
+    int NA, NB, N, M, Mk;
+    int m[3];
+    double L[3];
+    fft_plan1d P[3];
+    std::vector<double> g;     // half spectrum, reference index k2 + (m2/2+1)(K1 + m1 K0)
+    std::vector<double> gfull; // full spectrum, row-major (K0,K1,K2)
+    std::vector<cplx> z;
+};
+
+// src/step.cc:69-89 (update_g_lookup)
+static void step_g_lookup(orc_step& s) {
+    const int* m = s.m;
+    const double* L = s.L;
+    int N = s.NA + s.NB;
+    for (int k0 = -(m[0] - 1) / 2; k0 <= m[0] / 2; k0++) {
+        int K0 = (k0 < 0) ? (k0 + m[0]) : k0;
+        double kx_sq = k0 * k0 / (L[0] * L[0]);
+        for (int k1 = -(m[1] - 1) / 2; k1 <= m[1] / 2; k1++) {
+            int K1 = (k1 < 0) ? (k1 + m[1]) : k1;
+            double ky_sq = k1 * k1 / (L[1] * L[1]);
+            for (int k2 = 0; k2 <= m[2] / 2; k2++) {
+                double kz_sq = k2 * k2 / (L[2] * L[2]);
+                int k = k2 + (m[2] / 2 + 1) * (K1 + m[1] * K0);
+                double K = 2 * M_PI * pow(kx_sq + ky_sq + kz_sq, 0.5);
+                s.g[k] = exp(-K * K / (6.0 * N)) / s.M;
+            }
+        }
+    }
+    // Full-spectrum view used by the complex two-for-one transform: the value at
+    // K2 > m2/2 is the value at the conjugate point, which depends only on squares.
+    const int mk2 = m[2] / 2 + 1;
+    for (int K0 = 0; K0 < m[0]; K0++)
+        for (int K1 = 0; K1 < m[1]; K1++)
+            for (int K2 = 0; K2 < m[2]; K2++) {
+                int k2 = (K2 <= m[2] / 2) ? K2 : m[2] - K2;
+                s.gfull[((long)K0 * m[1] + K1) * m[2] + K2] = s.g[k2 + mk2 * (K1 + m[1] * K0)];
+            }
+}
+
+static void step_init(orc_step& s, int NA, int NB, const int* m, const double* L) {
+    s.NA = NA; s.NB = NB; s.N = NA + NB;
+    for (int d = 0; d < 3; d++) { s.m[d] = m[d]; s.L[d] = L[d]; make_plan(s.P[d], m[d]); }
+    s.M = m[0] * m[1] * m[2];
+    s.Mk = m[0] * m[1] * (m[2] / 2 + 1);
+    s.g.assign(s.Mk, 0.0);
+    s.gfull.assign(s.M, 0.0);
+    s.z.assign(s.M, cplx(0, 0));
+    step_g_lookup(s);
+}
+
+// src/step.cc:35-65 (step::fwd): q_out = h_sel * c2r(g * r2c(q_in)) for both halves
+static void step_fwd(orc_step& s, const double* q_in, double* q_out, const double* h, int i) {
+    const int M = s.M;
+    for (int r = 0; r < M; r++) s.z[r] = cplx(q_in[r], q_in[r + M]);
+    fft_3d(s.P, s.m, s.z.data(), -1);
+    for (int r = 0; r < M; r++) s.z[r] *= s.gfull[r];
+    fft_3d(s.P, s.m, s.z.data(), +1);
+    const double* hq = (i < s.NA) ? h : h + M;
+    const double* hqd = (i < s.NB) ? h + M : h;
+    for (int r = 0; r < M; r++) {
+        q_out[r] = s.z[r].real() * hq[r];
+        q_out[r + M] = s.z[r].imag() * hqd[r];
+    }
+}
+
+// -------------------------------------------------------------------------------------
+// diblock (src/diblock.cc:9-95)
+// -------------------------------------------------------------------------------------
+struct orc_diblock {
+    int M, NA, N;
+    std::vector<double> qr;  // 2(N+1)M; pair i at offset 2iM: [q_i | q+_{N+1-i}]
+    std::vector<double> h;   // [hA | hB]
+    orc_step st;
+    double* q1(int i) { return qr.data() + 2L * i * M; }
+    double* q2(int j) { return qr.data() + (2L * (N + 1 - j) + 1) * M; }
+};
+
+static double diblock_calc_concs(orc_diblock& d, double* w) {
+    const int M = d.M, N = d.N, NA = d.NA;
+    double* phiA = w + 2 * M;
+    double* phiB = w + 3 * M;
+    double* hA = d.h.data();
+    double* hB = d.h.data() + M;
+    // prepare_h (src/diblock.cc:64-72)
+    for (int r = 0; r < M; r++) {
+        hA[r] = exp(-(w[r] + w[r + M]) / N);
+        hB[r] = exp(-(-w[r] + w[r + M]) / N);
+    }
+    // q_1 = hA, q+_N = hB (src/diblock.cc:40)
+    for (int r = 0; r < 2 * M; r++) d.q1(1)[r] = d.h[r];
+    // chain (src/diblock.cc:43)
+    for (int i = 1; i < N; i++) step_fwd(d.st, d.q1(i), d.q1(i + 1), d.h.data(), i);
+    // Q (src/diblock.cc:46-48)
+    double Q = 0.0;
+    for (int r = 0; r < M; r++) Q += d.q1(N)[r];
+    Q /= M;
+    // phi sums (src/diblock.cc:51-53, 76-80)
+    for (int r = 0; r < 2 * M; r++) phiA[r] = 0.0;
+    for (int i = 1; i <= NA; i++) {
+        const double* a = d.q1(i); const double* b = d.q2(i);
+        for (int r = 0; r < M; r++) phiA[r] += a[r] * b[r];
+    }
+    for (int i = NA + 1; i <= N; i++) {
+        const double* a = d.q1(i); const double* b = d.q2(i);
+        for (int r = 0; r < M; r++) phiB[r] += a[r] * b[r];
+    }
+    // normalize_phi (src/diblock.cc:85-95)
+    for (int r = 0; r < M; r++) {
+        double pA = phiA[r] / (N * Q * hA[r]);
+        double pB = phiB[r] / (N * Q * hB[r]);
+        phiA[r] = pA - pB;
+        phiB[r] = pA + pB;
+    }
+    return log(Q);
+}
+
+// -------------------------------------------------------------------------------------
+// LU with partial pivoting (stands in for GSL, src/anderson.cc:106-127): row of the
+// largest |a_ij| in the column is swapped up; L unit lower, U upper; then solve.
+// -------------------------------------------------------------------------------------
+static void lu_solve(int n, const double* A_in, const double* b, double* x) {
+    std::vector<double> A(A_in, A_in + (size_t)n * n);
+    std::vector<int> p(n);
+    for (int i = 0; i < n; i++) p[i] = i;
+    for (int j = 0; j < n - 1; j++) {
+        int ip = j;
+        double amax = fabs(A[(size_t)j * n + j]);
+        for (int i = j + 1; i < n; i++) {
+            double v = fabs(A[(size_t)i * n + j]);
+            if (v > amax) { amax = v; ip = i; }
+        }
+        if (ip != j) {
+            for (int k = 0; k < n; k++) std::swap(A[(size_t)j * n + k], A[(size_t)ip * n + k]);
+            std::swap(p[j], p[ip]);
+        }
+        double ajj = A[(size_t)j * n + j];
+        if (ajj != 0.0) {
+            for (int i = j + 1; i < n; i++) {
+                double aij = A[(size_t)i * n + j] / ajj;
+                A[(size_t)i * n + j] = aij;
+                for (int k = j + 1; k < n; k++) A[(size_t)i * n + k] -= aij * A[(size_t)j * n + k];
+            }
+        }
+    }
+    std::vector<double> y(n);
+    for (int i = 0; i < n; i++) {
+        double s = b[p[i]];
+        for (int k = 0; k < i; k++) s -= A[(size_t)i * n + k] * y[k];
+        y[i] = s;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double s = y[i];
+        for (int k = i + 1; k < n; k++) s -= A[(size_t)i * n + k] * x[k];
+        x[i] = s / A[(size_t)i * n + i];
+    }
+}
+
+// -------------------------------------------------------------------------------------
+// anderson (src/anderson.cc:8-93)
+// -------------------------------------------------------------------------------------
+struct orc_anderson {
+    int M, nhMax;
+    std::vector<double> DD;   // (nhMax+1)^2, DD[a*(nhMax+1)+b]
+    std::vector<double> U, V, C;
+    std::vector<double> Dh, wh; // (nhMax+1) x M
+    double last_err = 0.0;
+};
+
+static int anderson_mix(orc_anderson& a, orc_diblock* dbc, int maxIter, double errTol, double* w) {
+    const int M = a.M, H = a.nhMax + 1;
+    double lambda, err = 1.0, S1;
+    int k, m, n, nh;
+    auto DD = [&](int i, int j) -> double& { return a.DD[(size_t)i * H + j]; };
+    auto Dh = [&](int i) { return a.Dh.data() + (size_t)i * M; };
+    auto wh = [&](int i) { return a.wh.data() + (size_t)i * M; };
+    for (k = 1; k < maxIter && err > errTol; k++) {
+        diblock_calc_concs(*dbc, w);
+        for (int r = 0; r < M; r++) { Dh(0)[r] = w[r + 3 * M] - 1.0; wh(0)[r] = w[r + M]; }
+        S1 = 0.0;
+        for (int r = 0; r < M; r++) S1 += pow(Dh(0)[r], 2.0);
+        err = pow(S1 / M, 0.5);
+        lambda = 1.0 - pow(0.9, double(k));
+        nh = (k < a.nhMax + 1) ? k - 1 : a.nhMax;
+        for (m = 0; m <= nh; m++) {
+            double s = 0.0;
+            for (int r = 0; r < M; r++) s += Dh(0)[r] * Dh(m)[r];
+            DD(0, m) = s;
+        }
+        if (k < 2) {
+            for (int r = 0; r < M; r++) w[r + M] += lambda * Dh(0)[r];
+        } else {
+            for (m = 1; m <= nh; m++) {
+                a.V[m - 1] = DD(0, 0) - DD(0, m);
+                for (n = 1; n <= m; n++)
+                    a.U[(m - 1) * nh + n - 1] = a.U[(n - 1) * nh + m - 1] =
+                        DD(0, 0) - DD(0, m) - DD(0, n) + DD(n, m);
+            }
+            lu_solve(nh, a.U.data(), a.V.data(), a.C.data());
+            for (int r = 0; r < M; r++) w[r + M] += lambda * Dh(0)[r];
+            for (n = 1; n <= nh; n++) {
+                const double* dn = Dh(n); const double* wn = wh(n);
+                const double* d0 = Dh(0); const double* w0 = wh(0);
+                for (int r = 0; r < M; r++)
+                    w[r + M] += a.C[n - 1] * ((wn[r] + lambda * dn[r]) - (w0[r] + lambda * d0[r]));
+            }
+        }
+        n = 1 + (k - 1) % (a.nhMax);
+        std::memcpy(Dh(n), Dh(0), sizeof(double) * M);
+        std::memcpy(wh(n), wh(0), sizeof(double) * M);
+        DD(n, n) = DD(0, 0);
+        for (m = 1; m < n; m++) DD(m, n) = DD(0, m);
+        for (m = n + 1; m <= nh; m++) DD(n, m) = DD(0, m);
+    }
+    a.last_err = err;
+    return k;
+}
+
+// -------------------------------------------------------------------------------------
+// langevin update (src/langevin.cc:23-37); the noise arrays come from the caller
+// -------------------------------------------------------------------------------------
+static void langevin_update(double* w, const double* eta_prev, const double* eta_new, int M,
+                            double XbN, double dt) {
+    for (int r = 0; r < M; r++)
+        w[r] += -(w[r + 2 * M] + 2 * w[r] / XbN) * dt + 0.5 * (eta_prev[r] + eta_new[r]);
+}
+
+// -------------------------------------------------------------------------------------
+// strFunc (src/strFunc.cc:9-103)
+// -------------------------------------------------------------------------------------
+struct orc_strfunc {
+    int m[3], M, Mk;
+    double dK, coeff, chi_b;
+    int nsamples = 0;
+    std::vector<double> S, K;
+    std::vector<int> wt, P;
+    fft_plan1d PL[3];
+    std::vector<cplx> z;
+};
+
+static void sk_init(orc_strfunc& s, const int* m, const double* L, double CV, double chi_b, double dK) {
+    for (int d = 0; d < 3; d++) { s.m[d] = m[d]; make_plan(s.PL[d], m[d]); }
+    s.M = m[0] * m[1] * m[2];
+    s.Mk = m[0] * m[1] * (m[2] / 2 + 1);
+    s.dK = dK;
+    s.chi_b = chi_b;
+    s.nsamples = 0;
+    s.coeff = CV / (chi_b * chi_b * s.M * s.M);
+    s.S.assign(s.Mk, 0.0);
+    s.K.assign(s.Mk, 0.0);
+    s.wt.assign(s.Mk, 2);
+    s.P.resize(s.Mk);
+    s.z.resize(s.M);
+    // calcK (src/strFunc.cc:80-103)
+    for (int k0 = -(m[0] - 1) / 2; k0 <= m[0] / 2; k0++) {
+        int K0 = (k0 < 0) ? (k0 + m[0]) : k0;
+        double kx_sq = k0 * k0 / (L[0] * L[0]);
+        for (int k1 = -(m[1] - 1) / 2; k1 <= m[1] / 2; k1++) {
+            int K1 = (k1 < 0) ? (k1 + m[1]) : k1;
+            double ky_sq = k1 * k1 / (L[1] * L[1]);
+            for (int k2 = 0; k2 <= m[2] / 2; k2++) {
+                double kz_sq = k2 * k2 / (L[2] * L[2]);
+                int k = k2 + (m[2] / 2 + 1) * (K1 + m[1] * K0);
+                s.K[k] = 2 * M_PI * pow(kx_sq + ky_sq + kz_sq, 0.5);
+                if ((k2 == 0) || (k2 == m[2] / 2)) s.wt[k] = 1;
+            }
+        }
+    }
+    std::iota(s.P.begin(), s.P.end(), 0);
+    const double* Kp = s.K.data();
+    std::stable_sort(s.P.begin(), s.P.end(), [Kp](size_t i, size_t j) { return Kp[i] < Kp[j]; });
+}
+
+// src/strFunc.cc:36-46: S_k += |r2c(W-)_k|^2 over the half spectrum
+static void sk_sample(orc_strfunc& s, const double* wm) {
+    for (int r = 0; r < s.M; r++) s.z[r] = cplx(wm[r], 0.0);
+    fft_3d(s.PL, s.m, s.z.data(), -1);
+    const int mk2 = s.m[2] / 2 + 1;
+    for (int K0 = 0; K0 < s.m[0]; K0++)
+        for (int K1 = 0; K1 < s.m[1]; K1++)
+            for (int k2 = 0; k2 < mk2; k2++)
+                s.S[k2 + mk2 * (K1 + s.m[1] * K0)] += std::norm(s.z[((long)K0 * s.m[1] + K1) * s.m[2] + k2]);
+    s.nsamples++;
+}
+
+// src/strFunc.cc:50-76
+static void sk_save(orc_strfunc& s, const std::string& fileName, int dp) {
+    double S_sum = 0.0;
+    int n_same = 0;
+    std::ofstream out(fileName);
+    out.precision(dp);
+    out.setf(std::ios::fixed, std::ios::floatfield);
+    for (int k = 0; k < s.Mk; k++) {
+        S_sum += s.wt[s.P[k]] * ((s.coeff / s.nsamples) * s.S[s.P[k]] - 0.5 / s.chi_b);
+        n_same += s.wt[s.P[k]];
+        if ((k == s.Mk - 1) || (fabs(s.K[s.P[k + 1]] - s.K[s.P[k]]) > s.dK)) {
+            out << s.K[s.P[k]] << "\t" << S_sum / n_same << std::endl;
+            S_sum = 0.0;
+            n_same = 0;
+        }
+    }
+}
+
+// -------------------------------------------------------------------------------------
+// parameters (src/lfts_params.cc:73-142)
+// -------------------------------------------------------------------------------------
+struct orc_params {
+    int N, NA, NB, isXeN, m[3], equil_its, sim_its, sample_freq, save_freq, loadType, M, Mk;
+    double chi_b, C, dt, L[3], V, sigma, XeN;
+};
+
+static double z_inf_discrete(const double* L, const int* m, int N, double Nbar, int tmax) {
+    double sum, prod = 1.0, X, ell, R0, R0dL[3];
+    sum = 0.5;
+    R0 = sqrt((double)N);
+    for (int i = 0; i < 3; i++) { R0dL[i] = R0 * L[i] / m[i]; prod *= R0dL[i]; }
+    ell = pow(prod, 1.0 / 3.0);
+    for (int t = 1; t <= tmax; t++) {
+        X = (M_PI / ell) * sqrt(t / 6.0);
+        prod = 1.0;
+        for (int i = 0; i < 3; i++) prod *= erf(X * ell / R0dL[i]);
+        sum += pow(sqrt(M_PI) / (2 * X), 3.0) * prod;
+    }
+    sum *= 2 * R0 / (pow(ell, 3.0) * sqrt(Nbar));
+    X = (M_PI / ell) * sqrt((0.5 + tmax) / 6.0);
+    return 1.0 - sum - 3 * R0 / (ell * sqrt(M_PI * Nbar) * X);
+}
+
+static bool read_params(orc_params& p, const std::string& fileName) {
+    std::ifstream in(fileName);
+    if (in.fail()) return false;
+    in >> p.N >> p.NA >> p.chi_b >> p.C >> p.dt >> p.isXeN;
+    in >> p.m[0] >> p.m[1] >> p.m[2] >> p.L[0] >> p.L[1] >> p.L[2];
+    in >> p.equil_its >> p.sim_its >> p.sample_freq >> p.save_freq >> p.loadType;
+    p.equil_its = (p.equil_its / p.sample_freq) * p.sample_freq;
+    p.sim_its = (p.sim_its / p.sample_freq) * p.sample_freq;
+    double z_inf = z_inf_discrete(p.L, p.m, p.N, p.C * p.C, 100);
+    if (p.isXeN == 1) { p.XeN = p.chi_b; p.chi_b = p.chi_b / z_inf; }
+    else p.XeN = p.chi_b * z_inf;
+    p.M = p.m[0] * p.m[1] * p.m[2];
+    p.Mk = p.m[0] * p.m[1] * (p.m[2] / 2 + 1);
+    p.V = p.L[0] * p.L[1] * p.L[2];
+    p.sigma = sqrt(2.0 * p.M * p.dt / (p.C * p.V));
+    p.NB = p.N - p.NA;
+    return true;
+}
+
+static void save_params(const orc_params& p, const std::string& fileName) {
+    double XN_out = (p.isXeN == 1) ? p.XeN : p.chi_b;
+    std::ofstream o(fileName);
+    o << p.N << " " << p.NA << " " << XN_out << " " << p.C << " " << p.dt << " " << p.isXeN << std::endl;
+    o << p.m[0] << " " << p.m[1] << " " << p.m[2] << " " << p.L[0] << " " << p.L[1] << " " << p.L[2] << std::endl;
+    o << p.equil_its << " " << p.sim_its << " " << p.sample_freq << " " << p.save_freq << " " << 1 << std::endl;
+}
+
+static void save_array(const double* a, const std::string& fileName, int n, bool append) {
+    std::ofstream o;
+    if (append) o.open(fileName, std::ios_base::app); else o.open(fileName);
+    for (int r = 0; r < n; r++) o << a[r] << std::endl;
+}
+
+static void read_array(double* a, const std::string& fileName, int n, int nIgnore) {
+    std::ifstream in(fileName);
+    for (int i = 0; i < nIgnore; i++) in.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+    for (int r = 0; r < n; r++) in >> a[r];
+}
+
+// -------------------------------------------------------------------------------------
+// simulation driver (src/lfts_simulation.cc:8-157)
+// -------------------------------------------------------------------------------------
+struct orc_sim {
+    orc_params p;
+    std::mt19937_64 rng;
+    std::normal_distribution<double> nrm;
+    std::vector<double> w, noise;
+    double* eta_prev;
+    double* eta_new;
+    std::unique_ptr<orc_diblock> dbc;
+    orc_anderson am;
+    orc_strfunc sk;
+    std::string outdir;
+    bool write_files = true;
+    bool verbose = false;
+    int last_nAM = 0;
+    std::vector<int> nAM_log;
+    std::string path(const std::string& f) const { return outdir.empty() ? f : outdir + "/" + f; }
+};
+
+static void sim_save_std(orc_sim& s, const std::string& f) {
+    save_params(s.p, s.path(f));
+    save_array(s.w.data(), s.path(f), 2 * s.p.M, true);
+}
+
+extern "C" {
+
+// ---- FFT and step -------------------------------------------------------------------
+void orc_fft1d(int n, double* data, int sign) {
+    fft_plan1d P; make_plan(P, n);
+    std::vector<cplx> scr(n);
+    fft_1d(P, reinterpret_cast<cplx*>(data), scr.data(), sign);
+}
+
+void orc_fft3d(const int* m, double* data, int sign) {
+    fft_plan1d P[3];
+    for (int d = 0; d < 3; d++) make_plan(P[d], m[d]);
+    fft_3d(P, m, reinterpret_cast<cplx*>(data), sign);
+}
+
+void* orc_create(int N, int NA, const int* m, const double* L) {
+    orc_diblock* d = new orc_diblock();
+    d->NA = NA; d->N = N;
+    d->M = m[0] * m[1] * m[2];
+    d->qr.assign(2L * (N + 1) * d->M, 0.0);
+    d->h.assign(2L * d->M, 0.0);
+    step_init(d->st, NA, N - NA, m, L);
+    return d;
+}
+
+void orc_destroy(void* h) { delete static_cast<orc_diblock*>(h); }
+
+void orc_ghalf(void* h, double* out) {
+    orc_diblock* d = static_cast<orc_diblock*>(h);
+    std::memcpy(out, d->st.g.data(), sizeof(double) * d->st.Mk);
+}
+
+double orc_calc_concs(void* h, double* w) { return diblock_calc_concs(*static_cast<orc_diblock*>(h), w); }
+
+// pair i (1..N) as stored by the reference: [q_i | q+_{N+1-i}]
+void orc_get_pair(void* h, int i, double* out) {
+    orc_diblock* d = static_cast<orc_diblock*>(h);
+    std::memcpy(out, d->q1(i), sizeof(double) * 2 * d->M);
+}
+
+void orc_step_fwd(void* h, const double* q_in, double* q_out, const double* hh, int i) {
+    orc_diblock* d = static_cast<orc_diblock*>(h);
+    step_fwd(d->st, q_in, q_out, hh, i);
+}
+
+// ---- Anderson ----------------------------------------------------------------------
+void* orc_am_create(int M, int nhMax) {
+    orc_anderson* a = new orc_anderson();
+    a->M = M; a->nhMax = nhMax;
+    a->DD.assign((size_t)(nhMax + 1) * (nhMax + 1), 0.0);
+    a->U.assign((size_t)nhMax * nhMax, 0.0);
+    a->V.assign(nhMax, 0.0);
+    a->C.assign(nhMax, 0.0);
+    a->Dh.assign((size_t)(nhMax + 1) * M, 0.0);
+    a->wh.assign((size_t)(nhMax + 1) * M, 0.0);
+    return a;
+}
+void orc_am_destroy(void* a) { delete static_cast<orc_anderson*>(a); }
+int orc_am_mix(void* a, void* dbc, int maxIter, double tol, double* w) {
+    return anderson_mix(*static_cast<orc_anderson*>(a), static_cast<orc_diblock*>(dbc), maxIter, tol, w);
+}
+double orc_am_last_err(void* a) { return static_cast<orc_anderson*>(a)->last_err; }
+
+void orc_lu_solve(int n, const double* A, const double* b, double* x) { lu_solve(n, A, b, x); }
+
+// ---- Langevin and noise --------------------------------------------------------------
+void orc_langevin_update(double* w, const double* eta_prev, const double* eta_new, int M, double XbN, double dt) {
+    langevin_update(w, eta_prev, eta_new, M, XbN, dt);
+}
+
+// n draws of std::normal_distribution<double>(0, sigma) from std::mt19937_64(seed)
+void orc_normals(unsigned long long seed, double sigma, int n, double* out) {
+    std::mt19937_64 r; r.seed(seed);
+    std::normal_distribution<double> d(0.0, sigma);
+    for (int i = 0; i < n; i++) out[i] = d(r);
+}
+void orc_mt64(unsigned long long seed, int n, unsigned long long* out) {
+    std::mt19937_64 r; r.seed(seed);
+    for (int i = 0; i < n; i++) out[i] = r();
+}
+
+// ---- structure function ------------------------------------------------------------
+void* orc_sk_create(const int* m, const double* L, double CV, double chi_b, double dK) {
+    orc_strfunc* s = new orc_strfunc();
+    sk_init(*s, m, L, CV, chi_b, dK);
+    return s;
+}
+void orc_sk_destroy(void* s) { delete static_cast<orc_strfunc*>(s); }
+void orc_sk_sample(void* s, const double* wm) { sk_sample(*static_cast<orc_strfunc*>(s), wm); }
+void orc_sk_get(void* sp, double* S, double* K, int* wt, int* P, int* ns) {
+    orc_strfunc* s = static_cast<orc_strfunc*>(sp);
+    if (S) std::memcpy(S, s->S.data(), sizeof(double) * s->Mk);
+    if (K) std::memcpy(K, s->K.data(), sizeof(double) * s->Mk);
+    if (wt) std::memcpy(wt, s->wt.data(), sizeof(int) * s->Mk);
+    if (P) std::memcpy(P, s->P.data(), sizeof(int) * s->Mk);
+    if (ns) *ns = s->nsamples;
+}
+void orc_sk_save(void* s, const char* fileName, int dp) { sk_save(*static_cast<orc_strfunc*>(s), fileName, dp); }
+
+// ---- parameters --------------------------------------------------------------------
+double orc_z_inf(const double* L, const int* m, int N, double Nbar, int tmax) {
+    return z_inf_discrete(L, m, N, Nbar, tmax);
+}
+
+// ints: N NA NB isXeN m0 m1 m2 equil sim sample save loadType M Mk (14)
+// dbls: chi_b C dt L0 L1 L2 V sigma XeN (9)
+int orc_read_params(const char* fileName, int* ints, double* dbls) {
+    orc_params p;
+    if (!read_params(p, fileName)) return 1;
+    int iv[14] = {p.N, p.NA, p.NB, p.isXeN, p.m[0], p.m[1], p.m[2], p.equil_its, p.sim_its,
+                  p.sample_freq, p.save_freq, p.loadType, p.M, p.Mk};
+    double dv[9] = {p.chi_b, p.C, p.dt, p.L[0], p.L[1], p.L[2], p.V, p.sigma, p.XeN};
+    std::memcpy(ints, iv, sizeof iv);
+    std::memcpy(dbls, dv, sizeof dv);
+    return 0;
+}
+
+// ---- whole simulation --------------------------------------------------------------
+// src/lfts_simulation.cc:8-57 (constructor): init order and RNG consumption order
+void* orc_sim_create(const char* inputFile, const char* outdir, int write_files) {
+    orc_sim* s = new orc_sim();
+    if (!read_params(s->p, inputFile)) { delete s; return nullptr; }
+    s->outdir = outdir ? outdir : "";
+    s->write_files = write_files != 0;
+    const int M = s->p.M;
+    s->rng.seed(123456789);
+    s->w.assign(4L * M, 0.0);
+    s->dbc.reset(static_cast<orc_diblock*>(orc_create(s->p.N, s->p.NA, s->p.m, s->p.L)));
+    orc_anderson* a = static_cast<orc_anderson*>(orc_am_create(M, 10));
+    s->am = *a; delete a;
+    s->nrm = std::normal_distribution<double>(0.0, s->p.sigma);
+    s->noise.assign(2L * M, 0.0);
+    for (int r = 0; r < M; r++) s->noise[r] = s->nrm(s->rng);
+    s->eta_prev = s->noise.data();
+    s->eta_new = s->noise.data() + M;
+    sk_init(s->sk, s->p.m, s->p.L, s->p.C * s->p.V, s->p.chi_b, 1e-5);
+    if (s->p.loadType == 1) {
+        read_array(s->w.data(), inputFile, 2 * M, 3);
+    } else if (s->p.loadType == 2) {
+        // field_generator::create_lamellar (src/field_generator.h:11-24), kx=3
+        const int* m = s->p.m;
+        for (int x = 0; x < m[0]; x++)
+            for (int y = 0; y < m[1]; y++)
+                for (int z = 0; z < m[2]; z++) {
+                    int r = m[2] * (x * m[1] + y) + z;
+                    s->w[r] = s->p.chi_b * cos(2.0 * M_PI * (3 * ((double)x) / m[0] + 0 * ((double)y) / m[1] + 0 * ((double)z) / m[2]));
+                    s->w[r + M] = 0.0;
+                }
+    } else {
+        std::uniform_real_distribution<double> u(0.0, 1.0);
+        for (int r = 0; r < M; r++) { s->w[r] = s->p.chi_b * (u(s->rng) - 0.5); s->w[r + M] = 0.0; }
+    }
+    s->last_nAM = anderson_mix(s->am, s->dbc.get(), 200, 1e-4, s->w.data());
+    if (s->write_files) {
+        sim_save_std(*s, "w_0");
+        save_array(s->w.data() + 2 * M, s->path("phi_0"), 2 * M, false);
+    }
+    return s;
+}
+
+void orc_sim_destroy(void* s) { delete static_cast<orc_sim*>(s); }
+
+void orc_sim_get_w(void* sp, double* w) {
+    orc_sim* s = static_cast<orc_sim*>(sp);
+    std::memcpy(w, s->w.data(), sizeof(double) * 4 * s->p.M);
+}
+void orc_sim_set_w(void* sp, const double* w) {
+    orc_sim* s = static_cast<orc_sim*>(sp);
+    std::memcpy(s->w.data(), w, sizeof(double) * 4 * s->p.M);
+}
+int orc_sim_last_nAM(void* sp) { return static_cast<orc_sim*>(sp)->last_nAM; }
+
+// One Langevin step of equilibrate()/statistics() (src/lfts_simulation.cc:67-118),
+// without file output: step_wm, mix, calc_concs (lnQ), and S(k) sample if asked.
+double orc_sim_step(void* sp, int sample) {
+    orc_sim* s = static_cast<orc_sim*>(sp);
+    const int M = s->p.M;
+    for (int r = 0; r < M; r++) s->eta_new[r] = s->nrm(s->rng);
+    langevin_update(s->w.data(), s->eta_prev, s->eta_new, M, s->p.chi_b, s->p.dt);
+    std::swap(s->eta_prev, s->eta_new);
+    s->last_nAM = anderson_mix(s->am, s->dbc.get(), 200, 1e-4, s->w.data());
+    double lnQ = diblock_calc_concs(*s->dbc, s->w.data());
+    if (sample) sk_sample(s->sk, s->w.data());
+    return lnQ;
+}
+
+static void sim_loop(orc_sim* s, bool stats) {
+    const int M = s->p.M;
+    const int nits = stats ? s->p.sim_its : s->p.equil_its;
+    const std::string tag = stats ? "st_" : "eq_";
+    int it;
+    for (it = 1; it <= nits; it++) {
+        double lnQ = orc_sim_step(s, stats && (it % s->p.sample_freq == 0));
+        std::cout << "lnQ = " << lnQ << std::endl;
+        if (it % s->p.save_freq == 0 && s->write_files) {
+            sim_save_std(*s, "w_" + tag + std::to_string(it));
+            save_array(s->w.data() + 2 * M, s->path("phi_" + tag + std::to_string(it)), 2 * M, false);
+            if (stats) sk_save(s->sk, s->path("struct_st_" + std::to_string(it)), 8);
+        }
+    }
+    if (s->write_files) {
+        sim_save_std(*s, "w_" + tag + std::to_string(it - 1));
+        save_array(s->w.data() + 2 * M, s->path("phi_" + tag + std::to_string(it - 1)), 2 * M, false);
+        if (stats) sk_save(s->sk, s->path("struct_st_" + std::to_string(it - 1)), 8);
+    }
+}
+
+void orc_sim_equilibrate(void* sp) { sim_loop(static_cast<orc_sim*>(sp), false); }
+void orc_sim_statistics(void* sp) { sim_loop(static_cast<orc_sim*>(sp), true); }
+
+// src/lfts_simulation.cc:122-135
+double orc_sim_getH(void* sp) {
+    orc_sim* s = static_cast<orc_sim*>(sp);
+    const int M = s->p.M;
+    double lnQ = diblock_calc_concs(*s->dbc, s->w.data());
+    double wp_sum = 0.0, wm2_sum = 0.0;
+    for (int r = 0; r < M; r++) wp_sum += s->w[r + M];
+    for (int r = 0; r < M; r++) wm2_sum += pow(s->w[r], 2.0);
+    return -lnQ + (wm2_sum / s->p.chi_b - wp_sum) / M;
+}
+
+void orc_sim_sk_get(void* sp, double* S, int* ns) {
+    orc_sim* s = static_cast<orc_sim*>(sp);
+    orc_sk_get(&s->sk, S, nullptr, nullptr, nullptr, ns);
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+"""GPU parity: the HIP path, called through the C ABI, against the oracle on the same
+seeded inputs. Tolerances (FP64): 1e-12 relative for one contour step, 1e-10 for a
+propagator evaluation, 1e-9 for a converged Anderson solve (its LU amplifies the
+last-bit differences of the Gram sums by the condition number of U)."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import lfts_cpu_amd as L
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MESHES = [
+    ((8, 8, 8), (3.0, 3.0, 3.0), 12, 6),
+    ((6, 5, 8), (2.5, 2.0, 3.5), 10, 3),     # radices 2, 3, 5 and an odd axis
+    ((16, 12, 10), (3.3, 2.9, 2.5), 16, 9),  # 12 = 4*3, 10 = 2*5
+    ((40, 40, 40), (4.38, 4.38, 4.38), 8, 4),  # the reference's 40^3 example mesh
+    ((7, 9, 14), (2.1, 2.7, 3.9), 9, 5),     # radix 7 and 9 = 3*3
+    ((32, 32, 32), (4.0, 4.0, 4.0), 64, 32),
+]
+
+
+def _fields(M, seed, a=2.0, b=1.0):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(4 * M)
+    w[:M] = rng.normal(0, a, M)
+    w[M:2 * M] = rng.normal(0, b, M)
+    return w
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("m,Lb,N,NA", MESHES)
+def test_contour_step(gpu, m, Lb, N, NA):
+    M = int(np.prod(m))
+    rng = np.random.default_rng(M)
+    qin = rng.uniform(0.2, 2.0, 2 * M)
+    h = rng.uniform(0.5, 1.5, 2 * M)
+    ctx = L.Context(N, NA, m, Lb)
+    d = O.Diblock(N, NA, m, Lb)
+    for i in (1, NA - 1, NA, N - NA, N - 1):
+        i = max(1, min(N - 1, i))
+        got = ctx.step_fwd(qin, h, i)
+        ref = d.step_fwd(qin, h, i)
+        assert rel(got[:M], ref[:M]) < 1e-12 and rel(got[M:], ref[M:]) < 1e-12, i
+
+
+@pytest.mark.parametrize("m,Lb,N,NA", MESHES)
+def test_calc_concs(gpu, m, Lb, N, NA):
+    M = int(np.prod(m))
+    w = _fields(M, 3)
+    ctx = L.Context(N, NA, m, Lb)
+    ctx.set_fields(w[:2 * M])
+    lnq = ctx.calc_concs()
+    got = ctx.get_fields(4)
+    d = O.Diblock(N, NA, m, Lb)
+    wo = w.copy()
+    lnq_ref = d.calc_concs(wo)
+    assert abs(lnq - lnq_ref) <= 1e-10 * max(1.0, abs(lnq_ref))
+    assert rel(got[2 * M:3 * M], wo[2 * M:3 * M]) < 1e-10
+    assert rel(got[3 * M:], wo[3 * M:]) < 1e-10
+    for i in (1, 2, N // 2, N):
+        assert rel(ctx.pair(i), d.pair(i)) < 1e-10, i
+    # exact invariant: mean(phi+) = 1
+    assert abs(got[3 * M:].mean() - 1.0) < 1e-12
+
+
+def test_calc_concs_closed_forms(gpu):
+    m, Lb, N, NA = (8, 6, 10), (3.0, 2.5, 3.5), 14, 5
+    M = int(np.prod(m))
+    ctx = L.Context(N, NA, m, Lb)
+    a = 0.8
+    w = np.zeros(2 * M)
+    w[:M] = a
+    ctx.set_fields(w)
+    assert abs(ctx.calc_concs() + a * (NA - (N - NA)) / N) < 1e-13
+    phi = ctx.get_fields(4)
+    assert np.allclose(phi[3 * M:], 1.0, rtol=0, atol=1e-12)
+    assert np.allclose(phi[2 * M:3 * M], (2 * NA - N) / N, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("m,Lb,N,NA", MESHES[:4])
+def test_anderson_mix(gpu, m, Lb, N, NA):
+    M = int(np.prod(m))
+    w = _fields(M, 4, 3.0, 1.0)
+    ctx = L.Context(N, NA, m, Lb)
+    ctx.set_fields(w[:2 * M])
+    k, err = ctx.anderson_mix(200, 1e-4)
+    got = ctx.get_fields(4)
+    d = O.Diblock(N, NA, m, Lb)
+    am = O.Anderson(M, 10)
+    wo = w.copy()
+    k_ref = am.mix(d, 200, 1e-4, wo)
+    assert k == k_ref
+    assert rel(got[M:2 * M], wo[M:2 * M]) < 1e-9
+    assert rel(got[2 * M:], wo[2 * M:]) < 1e-9
+    assert math.isclose(err, am.last_err(), rel_tol=1e-6)
+
+
+def test_langevin_step_host_and_device_noise(gpu):
+    import torch
+
+    m, Lb, N, NA = (8, 6, 10), (3.0, 2.5, 3.5), 12, 6
+    M = int(np.prod(m))
+    w = _fields(M, 5)
+    w[2 * M:] = np.random.default_rng(6).normal(0, 0.3, 2 * M)
+    e0, e1, e2 = (np.random.default_rng(s).normal(0, 1.5, M) for s in (7, 8, 9))
+    XbN, dt = 17.0, 0.5
+    ref = w.copy()
+    O.langevin_update(ref, e0, e1, XbN, dt)
+    O.langevin_update(ref, e1, e2, XbN, dt)
+    for dev in (False, True):
+        ctx = L.Context(N, NA, m, Lb)
+        ctx.set_fields(w)
+        ctx.langevin_init(e0)
+        if dev:
+            t1 = torch.from_numpy(e1).cuda()
+            t2 = torch.from_numpy(e2).cuda()
+            torch.cuda.synchronize()
+            ctx.langevin_step_dev(t1.data_ptr(), XbN, dt)
+            ctx.langevin_step_dev(t2.data_ptr(), XbN, dt)
+        else:
+            ctx.langevin_step(e1, XbN, dt)
+            ctx.langevin_step(e2, XbN, dt)
+        got = ctx.get_fields(4)
+        assert rel(got[:M], ref[:M]) < 1e-14
+        assert np.array_equal(got[M:], w[M:])
+
+
+@pytest.mark.parametrize("m,Lb", [((8, 6, 10), (3.0, 2.5, 4.0)), ((40, 40, 40), (4.38, 4.38, 4.38)),
+                                  ((7, 9, 5), (2.0, 2.0, 2.0))])
+def test_structure_function(gpu, m, Lb):
+    M = int(np.prod(m))
+    ctx = L.Context(10, 5, m, Lb)
+    s = O.StrFunc(m, Lb, 100.0, 20.0)
+    for seed in (1, 2, 3):
+        w = _fields(M, seed)
+        ctx.set_fields(w[:2 * M])
+        ctx.sk_sample()
+        s.sample(w[:M])
+    S, ns = ctx.sk_get()
+    So, _, _, _, nso = s.get()
+    assert ns == nso == 3
+    assert rel(S, So) < 1e-12
+
+
+def test_hamiltonian_closed_form(gpu):
+    m, Lb, N, NA, XbN = (4, 6, 8), (2.0, 2.5, 3.0), 10, 4, 15.0
+    M = int(np.prod(m))
+    a, c = 0.6, 0.25
+    ctx = L.Context(N, NA, m, Lb)
+    ctx.set_fields(np.concatenate([np.full(M, a), np.full(M, c)]))
+    H = ctx.hamiltonian(XbN)
+    assert math.isclose(H, a * (2 * NA - N) / N + a * a / XbN, rel_tol=1e-12)
+
+
+def _write_input(path, header, w):
+    with open(path, "w") as f:
+        f.write(header)
+        for v in w:
+            f.write("%.17g\n" % v)
+
+
+@pytest.mark.parametrize("m,N,NA,XbN,steps", [((16, 16, 16), 16, 8, 14.0, 12), ((12, 10, 8), 20, 7, 18.0, 8)])
+def test_langevin_trajectory(gpu, tmp_path, m, N, NA, XbN, steps):
+    """Whole Langevin steps (noise from the reference's stream, Anderson to 1e-4, the
+    extra calc_concs) against the oracle's restatement of the driver loop."""
+    M = int(np.prod(m))
+    Lb = (3.2, 3.0, 2.8)
+    w0 = _fields(M, 10, 0.1, 0.1)[:2 * M]
+    f = tmp_path / "in"
+    _write_input(f, f"{N} {NA} {XbN} 50 0.5 0\n{m[0]} {m[1]} {m[2]} {Lb[0]} {Lb[1]} {Lb[2]}\n"
+                    f"{steps} 0 1 1000 1\n", w0)
+    sim = O.Sim(str(f))
+    p = sim.params
+    ctx = L.Context(N, NA, m, Lb)
+    ctx.set_fields(w0)
+    rng = L.Rng(123456789, p["sigma"])
+    ctx.langevin_init(rng.normals(M))
+    k0, _ = ctx.anderson_mix(200, 1e-4)
+    assert k0 == sim.last_nAM()
+    worst = 0.0
+    for step in range(steps):
+        ctx.langevin_step(rng.normals(M), p["chi_b"], p["dt"])
+        k, _ = ctx.anderson_mix(200, 1e-4)
+        lnq = ctx.calc_concs()
+        lnq_ref = sim.step()
+        assert k == sim.last_nAM(), step
+        wg, wo = ctx.get_fields(4), sim.w()
+        e = max(rel(wg[:M], wo[:M]), rel(wg[M:2 * M], wo[M:2 * M]), rel(wg[2 * M:], wo[2 * M:]),
+                abs(lnq - lnq_ref) / max(1.0, abs(lnq_ref)))
+        worst = max(worst, e)
+        assert e < 1e-8, (step, e)
+    print(f"worst relative difference over {steps} steps: {worst:.3e}")
+    H = ctx.hamiltonian(p["chi_b"])
+    assert math.isclose(H, sim.getH(), rel_tol=1e-8)
