@@ -1,0 +1,209 @@
+#!/usr/bin/env python
+"""Langevin-step throughput of the gfx950 path on one MI355X.
+
+Workload (the 64^3 plain FP64 case): diblock N=90, NA=45, XbN=17.0 (bare, isXeN=0),
+C=50, Ndt=0.5, mesh 64^3, box 4.4^3 R0; start W- = 17 cos(2 pi 3x/64) + N(0, 0.05^2)
+(numpy seed 2), w+ = 0; noise from the reference's stream (mt19937_64 seed 123456789,
+normal(0, sigma)); Anderson mixing to 1e-4 (maxIter 200, 10 histories).
+
+A step is one whole Langevin step of the reference's loop (src/lfts_simulation.cc:72-76):
+the W- update, the Anderson saddle-point solve, and the extra propagator evaluation
+that prints ln Q. The noise of every timed step is drawn on the host beforehand and is
+resident in HBM when timing starts.
+
+Prints one JSON line (rank 0). Extra detail goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Langevin steps/s at 64³,N=90 FP64 (plain vs batched); % HBM roofline"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def initial_fields(m, amp, sigma0, seed):
+    M = int(np.prod(m))
+    x = np.arange(m[0])[:, None, None] * np.ones(m)
+    wm = amp * np.cos(2.0 * np.pi * (3 * x / m[0]))
+    rng = np.random.default_rng(seed)
+    wm = wm.reshape(-1) + rng.normal(0.0, sigma0, M)
+    return np.concatenate([wm, np.zeros(M)])
+
+
+def algorithmic_bytes(M, N, nAM_list, with_langevin=True):
+    """Bytes of one Langevin step per the roofline accounting: each counted array is
+    touched once, FFT-internal passes excluded."""
+    B_PE = 8.0 * M * (8 * N - 2)
+    total = 0.0
+    for nAM in nAM_list:
+        b = (nAM + 1) * B_PE
+        for k in range(1, nAM + 1):
+            nh = min(k - 1, 10)
+            b += 8.0 * M * (3 * nh + 6)
+        if with_langevin:
+            b += 40.0 * M
+        total += b
+    return total
+
+
+def cpu_baseline(w_state, params, m, Lb, N, NA, seconds_hint=30.0):
+    """The oracle (CPU restatement, OpenMP over FFT lines) timed on this host for one
+    Langevin step of the same workload, started from the GPU's state after warm-up."""
+    import oracle as O
+
+    M = int(np.prod(m))
+    with tempfile.TemporaryDirectory() as td:
+        f = os.path.join(td, "input")
+        with open(f, "w") as fh:
+            fh.write(f"{N} {NA} {params['XbN']} {params['C']} {params['dt']} 0\n")
+            fh.write(f"{m[0]} {m[1]} {m[2]} {Lb[0]} {Lb[1]} {Lb[2]}\n")
+            fh.write("1 0 1 1000 1\n")
+            np.savetxt(fh, w_state[:2 * M], fmt="%.17g")
+        sim = O.Sim(f, initial_mix=False)
+        sim.set_w(w_state)
+        t0 = time.perf_counter()
+        sim.step(False)
+        dt = time.perf_counter() - t0
+        nAM = sim.last_nAM() - 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": 1.0 / dt,
+        "unit": "Langevin steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one Langevin step of the same 64^3 N=90 workload from the GPU state after warm-up "
+                  f"(n_AM={nAM}, {dt:.1f} s); OpenMP over FFT lines, element-wise loops serial",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mesh", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(replicas would shard one per GPU with no exchange per step).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import lfts_cpu_amd as L
+
+    N, NA, XbN, C, dt = 90, 45, 17.0, 50.0, 0.5
+    m = (args.mesh,) * 3
+    Lb = (4.4, 4.4, 4.4)
+    M = int(np.prod(m))
+    V = Lb[0] * Lb[1] * Lb[2]
+    sigma = math.sqrt(2.0 * M * dt / (C * V))
+    params = {"XbN": XbN, "C": C, "dt": dt}
+
+    w0 = initial_fields(m, XbN, 0.05, 2)
+    ctx = L.Context(N, NA, m, Lb)
+    ctx.set_fields(w0)
+    rng = L.Rng(123456789, sigma)
+    ctx.langevin_init(rng.normals(M))
+    t0 = time.perf_counter()
+    k0, err0 = ctx.anderson_mix(200, 1e-4)
+    log(f"initial mix: k={k0} err={err0:.3e} ({time.perf_counter() - t0:.2f} s)")
+
+    nsteps = args.warmup + args.steps
+    eta = torch.empty((nsteps, M), dtype=torch.float64, device="cuda")
+    for s in range(nsteps):
+        eta[s].copy_(torch.from_numpy(rng.normals(M)))
+    torch.cuda.synchronize()
+
+    for s in range(args.warmup):
+        lnq, k = ctx.full_step_dev(eta[s].data_ptr(), XbN, dt)
+        log(f"warmup {s}: lnQ={lnq:.10g} n_AM={k - 1}")
+    torch.cuda.synchronize()
+
+    ctx.profile(True)
+    nAM = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.warmup, nsteps):
+        lnq, k = ctx.full_step_dev(eta[s].data_ptr(), XbN, dt)
+        nAM.append(k - 1)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_chain, n_cs, ms_pe, n_pe = ctx.profile_read()
+    ctx.profile(False)
+    log(f"timed {args.steps} steps: {elapsed:.3f} s, n_AM={nAM}, lnQ={lnq:.10g}")
+
+    steps_per_s = args.steps / elapsed
+    cs_ms = ms_chain / max(n_cs, 1)
+    cs_bytes = 48.0 * M  # one contour step: read the q pair and h, write the q pair
+    achieved = cs_bytes / (cs_ms * 1e-3) / 1e9
+    step_bytes = algorithmic_bytes(M, N, nAM)
+    step_gbps = step_bytes / elapsed / 1e9
+    cs_per_s = sum((N - 1) * (n + 1) for n in nAM) * M / elapsed
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(ctx.get_fields(4), params, m, Lb, N, NA)
+        except Exception as e:  # the baseline is reported, not required
+            log(f"cpu baseline failed: {e!r}")
+
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "Langevin steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: lamellar(300) W- + N(0,0.05^2) (numpy seed 2), w+=0; noise from mt19937_64 seed 123456789",
+        "config": {
+            "workload": f"plain FP64 Langevin step, diblock N={N} NA={NA} XbN={XbN} (isXeN=0) C={C} Ndt={dt}, "
+                        f"mesh {m[0]}x{m[1]}x{m[2]}, box 4.4^3, Anderson tol 1e-4 (maxIter 200, 10 histories)",
+            "mesh": list(m),
+            "N": N,
+            "NA": NA,
+            "mean_n_AM": float(np.mean(nAM)),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": "contour step (48*M algorithmic bytes), HIP events on the context stream",
+            "contour_step_us": cs_ms * 1e3,
+        },
+        "step_algorithmic_GBps": step_gbps,
+        "step_roofline_frac": step_gbps / HBM_PEAK_GBPS,
+        "contour_steps_M_per_s": cs_per_s,
+        "propagator_eval_ms": ms_pe / max(n_pe, 1),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -105,6 +105,15 @@ int lfts_step_fwd(lfts_ctx* ctx, const double* q_in, double* q_out, const double
 /* Counters: propagator evaluations and contour steps run on this context so far. */
 int lfts_counters(lfts_ctx* ctx, long long* n_prop_evals, long long* n_contour_steps);
 
+/* Live timing with HIP events recorded on the context stream (no reference counterpart;
+ * the reference only prints whole-second wall times, src/fts_cpu.cc:44-57). While enabled,
+ * every propagator evaluation records the duration of its contour-step chain (the N-1
+ * contour steps, back to back) and of the whole evaluation. lfts_profile_read returns
+ * the sums (ms) and counts since the last enable. */
+int lfts_profile(lfts_ctx* ctx, int enable);
+int lfts_profile_read(lfts_ctx* ctx, double* ms_chain, long long* n_contour_steps, double* ms_pe,
+                      long long* n_prop_evals);
+
 /* Host random stream of the reference: std::mt19937_64 seeded by the caller (the
  * reference seeds 123456789, src/lfts_simulation.cc:77) feeding one persistent
  * std::normal_distribution<double>(0, sigma) (src/langevin.cc:13,16,28); uniforms use a

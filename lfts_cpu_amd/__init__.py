@@ -65,6 +65,8 @@ SIGNATURES = {
     "lfts_get_pair": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "lfts_step_fwd": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int]),
     "lfts_counters": (ctypes.c_int, [_vp, _llp, _llp]),
+    "lfts_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lfts_profile_read": (ctypes.c_int, [_vp, _dp, _llp, _dp, _llp]),
     "lfts_rng_create": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_double, ctypes.POINTER(_vp)]),
     "lfts_rng_normals": (ctypes.c_int, [_vp, _dp, ctypes.c_longlong]),
     "lfts_rng_uniforms": (ctypes.c_int, [_vp, _dp, ctypes.c_longlong]),
@@ -210,6 +212,16 @@ class Context:
         self._chk(lib().lfts_step_fwd(self.h, _d(np.ascontiguousarray(q_in)), _d(out),
                                       _d(np.ascontiguousarray(hh)), int(i)))
         return out
+
+    def profile(self, enable: bool = True):
+        self._chk(lib().lfts_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        """(ms in contour-step chains, contour steps, ms in whole evaluations, evaluations)"""
+        a, c = ctypes.c_double(), ctypes.c_double()
+        b, d = ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(lib().lfts_profile_read(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
 
     def counters(self):
         a = ctypes.c_longlong()

@@ -55,6 +55,13 @@ struct lfts_ctx {
     int nsamples = 0;
     int nblk = 1, nred = 1;
     long long n_pe = 0, n_cs = 0;
+    // live event timing (lfts_profile)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;  // free events
+    struct EvQuad { hipEvent_t pe0, ch0, ch1, pe1; int nsteps; };
+    std::vector<EvQuad> ev_pending;
+    double prof_ms_chain = 0.0, prof_ms_pe = 0.0;
+    long long prof_n_cs = 0, prof_n_pe = 0;
     std::vector<double> DD;   // (nhMax+1)^2
     std::vector<double> U, V, C;
     std::string err;
@@ -166,17 +173,56 @@ static int contour_step(lfts_ctx* c, const double* in, double* out, const double
     return LFTS_OK;
 }
 
+static int get_event(lfts_ctx* c, hipEvent_t* e) {
+    if (!c->ev_pool.empty()) {
+        *e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return LFTS_OK;
+    }
+    HIPCHK(c, hipEventCreate(e));
+    return LFTS_OK;
+}
+
+// Fold completed event quads into the profile sums (the stream has been synchronised).
+static int resolve_events(lfts_ctx* c) {
+    for (auto& q : c->ev_pending) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, q.ch0, q.ch1));
+        HIPCHK(c, hipEventElapsedTime(&b, q.pe0, q.pe1));
+        c->prof_ms_chain += a;
+        c->prof_ms_pe += b;
+        c->prof_n_cs += q.nsteps;
+        c->prof_n_pe += 1;
+        for (hipEvent_t e : {q.pe0, q.ch0, q.ch1, q.pe1}) c->ev_pool.push_back(e);
+    }
+    c->ev_pending.clear();
+    return LFTS_OK;
+}
+
 // src/diblock.cc:30-60, without the host read of Q; scal[0] = Q.
 static int calc_concs_async(lfts_ctx* c) {
     const long M = c->M;
+    lfts_ctx::EvQuad ev{};
+    if (c->prof) {
+        for (hipEvent_t* e : {&ev.pe0, &ev.ch0, &ev.ch1, &ev.pe1})
+            if (int rc = get_event(c, e)) return rc;
+        HIPCHK(c, hipEventRecord(ev.pe0, c->st));
+    }
     HIPCHK(c, lfts::launch_prepare_h(c->w, c->h, M, c->N, c->nblk, c->st));
+    if (c->prof) HIPCHK(c, hipEventRecord(ev.ch0, c->st));
     for (int i = 1; i < c->N; i++) {
         int rc = contour_step(c, pair_ptr(c, i), pair_ptr(c, i + 1), c->h, i);
         if (rc) return rc;
     }
+    if (c->prof) HIPCHK(c, hipEventRecord(ev.ch1, c->st));
     HIPCHK(c, lfts::launch_sum_partial(pair_ptr(c, c->N), M, c->part, c->nred, c->st));
     HIPCHK(c, lfts::launch_reduce_final(c->part, c->nred, 1, (double)M, c->scal, c->st));
     HIPCHK(c, lfts::launch_phi(c->h, c->q, c->w, c->scal, M, c->N, c->NA, c->nblk, c->st));
+    if (c->prof) {
+        HIPCHK(c, hipEventRecord(ev.pe1, c->st));
+        ev.nsteps = c->N - 1;
+        c->ev_pending.push_back(ev);
+    }
     c->n_pe++;
     return LFTS_OK;
 }
@@ -185,6 +231,7 @@ static int read_scalars(lfts_ctx* c, int n, double* out) {
     HIPCHK(c, hipMemcpyAsync(c->pinned, c->scal, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     std::memcpy(out, c->pinned, sizeof(double) * n);
+    if (!c->ev_pending.empty()) return resolve_events(c);
     return LFTS_OK;
 }
 
@@ -295,6 +342,9 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
 int lfts_destroy(lfts_ctx* c) {
     if (!c) return LFTS_EINVAL;
     if (c->st) hipStreamSynchronize(c->st);
+    for (auto& q : c->ev_pending)
+        for (hipEvent_t e : {q.pe0, q.ch0, q.ch1, q.pe1}) c->ev_pool.push_back(e);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     for (auto& p : c->ax) {
         hipFree(p.tw);
         hipFree(p.gperm);
@@ -551,6 +601,27 @@ int lfts_step_fwd(lfts_ctx* c, const double* q_in, double* q_out, const double* 
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(q_out, c->test + 4 * M, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    return LFTS_OK;
+}
+
+int lfts_profile(lfts_ctx* c, int enable) {
+    if (check_ctx(c)) return LFTS_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (int rc = resolve_events(c)) return rc;
+    c->prof = enable != 0;
+    c->prof_ms_chain = c->prof_ms_pe = 0.0;
+    c->prof_n_cs = c->prof_n_pe = 0;
+    return LFTS_OK;
+}
+
+int lfts_profile_read(lfts_ctx* c, double* ms_chain, long long* n_cs, double* ms_pe, long long* n_pe) {
+    if (check_ctx(c)) return LFTS_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (int rc = resolve_events(c)) return rc;
+    if (ms_chain) *ms_chain = c->prof_ms_chain;
+    if (n_cs) *n_cs = c->prof_n_cs;
+    if (ms_pe) *ms_pe = c->prof_ms_pe;
+    if (n_pe) *n_pe = c->prof_n_pe;
     return LFTS_OK;
 }
 

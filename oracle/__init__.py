@@ -237,10 +237,11 @@ def read_params(path: str) -> dict:
 class Sim:
     """lfts_simulation restatement (src/lfts_simulation.cc)."""
 
-    def __init__(self, input_file: str, outdir: str = "", write_files: bool = False):
+    def __init__(self, input_file: str, outdir: str = "", write_files: bool = False, initial_mix: bool = True):
         self.params = read_params(input_file)
         self.M = self.params["M"]
-        self.h = lib().orc_sim_create(input_file.encode(), outdir.encode(), int(write_files))
+        flags = int(write_files) | (0 if initial_mix else 2)
+        self.h = lib().orc_sim_create(input_file.encode(), outdir.encode(), flags)
         if not self.h:
             raise FileNotFoundError(input_file)
 

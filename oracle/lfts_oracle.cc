@@ -659,8 +659,11 @@ int orc_read_params(const char* fileName, int* ints, double* dbls) {
 }
 
 // ---- whole simulation --------------------------------------------------------------
-// src/lfts_simulation.cc:8-57 (constructor): init order and RNG consumption order
-void* orc_sim_create(const char* inputFile, const char* outdir, int write_files) {
+// src/lfts_simulation.cc:8-57 (constructor): init order and RNG consumption order.
+// flags: bit 0 writes the output files; bit 1 skips the initial mix (for timing a
+// Langevin step from a state that is set afterwards with orc_sim_set_w).
+void* orc_sim_create(const char* inputFile, const char* outdir, int flags) {
+    const int write_files = flags & 1;
     orc_sim* s = new orc_sim();
     if (!read_params(s->p, inputFile)) { delete s; return nullptr; }
     s->outdir = outdir ? outdir : "";
@@ -693,6 +696,7 @@ void* orc_sim_create(const char* inputFile, const char* outdir, int write_files)
         std::uniform_real_distribution<double> u(0.0, 1.0);
         for (int r = 0; r < M; r++) { s->w[r] = s->p.chi_b * (u(s->rng) - 0.5); s->w[r + M] = 0.0; }
     }
+    if (flags & 2) return s;
     s->last_nAM = anderson_mix(s->am, s->dbc.get(), 200, 1e-4, s->w.data());
     if (s->write_files) {
         sim_save_std(*s, "w_0");
