@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <numeric>
@@ -28,6 +29,7 @@ struct AxisPlan {
     double* gperm = nullptr;
     int* perm = nullptr;
     int T = 1, pitch = 2;
+    double* gnat = nullptr;  // kernel in natural frequency order (g(k)/n), for the fused path
 };
 
 }  // namespace
@@ -42,6 +44,8 @@ struct lfts_ctx {
     double* h = nullptr;      // 2M: hA, hB (= pair 1)
     double* q = nullptr;      // (N-1) pairs for beads 2..N
     double* tmp = nullptr;    // 2M
+    double* tmp2 = nullptr;   // 2M (second partial-convolution buffer of the fused path)
+    bool fused = false;       // one launch per contour step (slab passes)
     double* Dh = nullptr;     // (nhMax+1) M
     double* wh = nullptr;     // (nhMax+1) M
     double* noise = nullptr;  // 2M: eta_prev | staging for eta_new
@@ -120,12 +124,16 @@ static int build_axis(lfts_ctx* c, AxisPlan& p, int n, double L, int N) {
         perm[k] = pos;
         gperm[pos] = std::exp(-K * K / (6.0 * N)) / (double)n;
     }
+    std::vector<double> gnat(n);
+    for (int k = 0; k < n; k++) gnat[k] = gperm[perm[k]];
     HIPCHK(c, hipMalloc(&p.tw, sizeof(double2) * n));
     HIPCHK(c, hipMalloc(&p.gperm, sizeof(double) * n));
     HIPCHK(c, hipMalloc(&p.perm, sizeof(int) * n));
+    HIPCHK(c, hipMalloc(&p.gnat, sizeof(double) * n));
     HIPCHK(c, hipMemcpy(p.tw, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(p.gperm, gperm.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(p.perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(p.gnat, gnat.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     p.T = std::max(1, std::min(16, 4096 / n));
     p.pitch = n + 1;
     return LFTS_OK;
@@ -199,6 +207,53 @@ static int resolve_events(lfts_ctx* c) {
     return LFTS_OK;
 }
 
+// The N-1 contour steps of src/diblock.cc:43 as alternating slab passes, one launch
+// per step: a prologue x-slab pass convolves q_1 along y and z, then pass i (i = 1..N-1)
+// finishes step i (y-slab for odd i, x-slab for even i), stores q_{i+1} and starts step
+// i+1.
+static int fused_chain(lfts_ctx* c) {
+    const long M = c->M;
+    const int n = c->d.m[0];
+    const long m1 = c->d.m[1], m2 = c->d.m[2];
+    double* T[2] = {c->tmp, c->tmp2};
+    auto setup = [&](lfts::SlabArgs& a, bool xslab) {
+        std::memset(&a, 0, sizeof a);
+        a.M = M;
+        a.tw = c->ax[2].tw;
+        a.gb = c->ax[2].gnat;
+        if (xslab) {  // plane x0: outer axis y
+            a.plane_stride = m1 * m2;
+            a.Sa = m2;
+            a.ga = c->ax[1].gnat;
+        } else {      // plane y0: outer axis x
+            a.plane_stride = m2;
+            a.Sa = m1 * m2;
+            a.ga = c->ax[0].gnat;
+        }
+    };
+    lfts::SlabArgs a;
+    setup(a, true);
+    a.Tin = c->h;  // pair 1 = [q_1 | q+_N] = [hA | hB]
+    a.Tout = T[0];
+    a.finish = 0;
+    a.cont = 1;
+    HIPCHK(c, lfts::launch_slab(a, n, c->d.m[0], c->st));
+    for (int i = 1; i < c->N; i++) {
+        const bool xslab = (i % 2 == 0);
+        setup(a, xslab);
+        a.Tin = T[(i - 1) & 1];
+        a.Tout = T[i & 1];
+        a.qout = pair_ptr(c, i + 1);
+        a.hq = (i < c->NA) ? c->h : c->h + M;      // q_{i+1}   (src/step.cc:53-57)
+        a.hqd = (i < c->NB) ? c->h + M : c->h;     // q+_{N-i} (src/step.cc:60-64)
+        a.finish = 1;
+        a.cont = (i < c->N - 1) ? 1 : 0;
+        HIPCHK(c, lfts::launch_slab(a, n, xslab ? c->d.m[0] : c->d.m[1], c->st));
+        c->n_cs++;
+    }
+    return LFTS_OK;
+}
+
 // src/diblock.cc:30-60, without the host read of Q; scal[0] = Q.
 static int calc_concs_async(lfts_ctx* c) {
     const long M = c->M;
@@ -210,9 +265,13 @@ static int calc_concs_async(lfts_ctx* c) {
     }
     HIPCHK(c, lfts::launch_prepare_h(c->w, c->h, M, c->N, c->nblk, c->st));
     if (c->prof) HIPCHK(c, hipEventRecord(ev.ch0, c->st));
-    for (int i = 1; i < c->N; i++) {
-        int rc = contour_step(c, pair_ptr(c, i), pair_ptr(c, i + 1), c->h, i);
-        if (rc) return rc;
+    if (c->fused) {
+        if (int rc = fused_chain(c)) return rc;
+    } else {
+        for (int i = 1; i < c->N; i++) {
+            int rc = contour_step(c, pair_ptr(c, i), pair_ptr(c, i + 1), c->h, i);
+            if (rc) return rc;
+        }
     }
     if (c->prof) HIPCHK(c, hipEventRecord(ev.ch1, c->st));
     HIPCHK(c, lfts::launch_sum_partial(pair_ptr(c, c->N), M, c->part, c->nred, c->st));
@@ -316,6 +375,14 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     CREATE_HIP(hipMalloc(&c->h, sizeof(double) * 2 * M));
     CREATE_HIP(hipMalloc(&c->q, sizeof(double) * 2 * M * (size_t)(c->N - 1)));
     CREATE_HIP(hipMalloc(&c->tmp, sizeof(double) * 2 * M));
+    {
+        // LFTS_PATH=generic selects the three-pass line kernels even where the fused
+        // slab pass applies (used to cross-check the two paths)
+        const char* path = std::getenv("LFTS_PATH");
+        const bool generic = path && std::strcmp(path, "generic") == 0;
+        c->fused = !generic && lfts::slab_supported(desc->m[0], desc->m[1], desc->m[2]);
+        if (c->fused) CREATE_HIP(hipMalloc(&c->tmp2, sizeof(double) * 2 * M));
+    }
     CREATE_HIP(hipMalloc(&c->Dh, sizeof(double) * M * (size_t)(c->nhMax + 1)));
     CREATE_HIP(hipMalloc(&c->wh, sizeof(double) * M * (size_t)(c->nhMax + 1)));
     CREATE_HIP(hipMalloc(&c->noise, sizeof(double) * 2 * M));
@@ -349,7 +416,9 @@ int lfts_destroy(lfts_ctx* c) {
         hipFree(p.tw);
         hipFree(p.gperm);
         hipFree(p.perm);
+        hipFree(p.gnat);
     }
+    hipFree(c->tmp2);
     hipFree(c->w); hipFree(c->h); hipFree(c->q); hipFree(c->tmp); hipFree(c->Dh); hipFree(c->wh);
     hipFree(c->noise); hipFree(c->S); hipFree(c->zk); hipFree(c->part); hipFree(c->scal); hipFree(c->test);
     if (c->pinned) hipHostFree(c->pinned);

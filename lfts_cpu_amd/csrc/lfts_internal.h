@@ -30,6 +30,26 @@ struct LineArgs {
     double* Sacc;
 };
 
+// Fused slab pass (one contour step per launch), cubic 32^3 / 64^3 meshes.
+struct SlabArgs {
+    const double* Tin;   // partially convolved pair (2 fields of M), or pair 1 for the prologue
+    double* Tout;        // partially convolved pair for the next pass
+    double* qout;        // stored propagator pair q_{i+1} (finishing passes)
+    const double* hq;    // Boltzmann factor for field 0 (q)
+    const double* hqd;   // Boltzmann factor for field 1 (q+)
+    const double* ga;    // kernel along the slab's outer axis (n entries, includes 1/n)
+    const double* gb;    // kernel along z
+    const double2* tw;   // exp(-2 pi i k/n), n entries
+    long M;              // field stride
+    long plane_stride;   // offset between consecutive slab planes
+    long Sa;             // stride of the outer axis inside a plane
+    int finish;          // complete the step in flight: Ca, *h, store q_{i+1}
+    int cont;            // start the next step: Ca, Cz, store Tout
+};
+
+bool slab_supported(int n0, int n1, int n2);
+hipError_t launch_slab(const SlabArgs& a, int n, int nplanes, hipStream_t st);
+
 struct AmCoef {
     double lambda;
     int nh, slot;

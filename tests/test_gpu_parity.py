@@ -200,3 +200,30 @@ def test_langevin_trajectory(gpu, tmp_path, m, N, NA, XbN, steps):
     print(f"worst relative difference over {steps} steps: {worst:.3e}")
     H = ctx.hamiltonian(p["chi_b"])
     assert math.isclose(H, sim.getH(), rel_tol=1e-8)
+
+
+@pytest.mark.parametrize("n,N,NA", [(64, 12, 5), (32, 20, 13), (64, 9, 9)])
+def test_fused_slab_path_against_oracle_and_generic_path(gpu, monkeypatch, n, N, NA):
+    """The one-launch-per-step slab path (cubic 32^3/64^3) against the oracle and the
+    three-pass line path on the same input."""
+    m, Lb = (n, n, n), (4.4, 4.4, 4.4)
+    M = n ** 3
+    w = _fields(M, 12, 3.0, 1.0)
+    ctx = L.Context(N, NA, m, Lb)
+    ctx.set_fields(w[:2 * M])
+    lnq = ctx.calc_concs()
+    got = ctx.get_fields(4)
+    monkeypatch.setenv("LFTS_PATH", "generic")
+    gen = L.Context(N, NA, m, Lb)
+    gen.set_fields(w[:2 * M])
+    lnq_gen = gen.calc_concs()
+    got_gen = gen.get_fields(4)
+    d = O.Diblock(N, NA, m, Lb)
+    wo = w.copy()
+    lnq_ref = d.calc_concs(wo)
+    for cand, wc in ((lnq, got), (lnq_gen, got_gen)):
+        assert abs(cand - lnq_ref) <= 1e-10 * max(1.0, abs(lnq_ref))
+        assert rel(wc[2 * M:], wo[2 * M:]) < 1e-10
+    for i in (2, N // 2 + 1, N):
+        assert rel(ctx.pair(i), d.pair(i)) < 1e-10, i
+        assert rel(ctx.pair(i), gen.pair(i)) < 1e-12, i
