@@ -90,14 +90,14 @@ __device__ __forceinline__ void dft(double2* v) {
 
 // 1D circular convolution of one complex line of length N = R1*R2 held by R2 threads
 // (this thread: index j, elements z[k] = x[j + R2 k]). lx is the line's LDS exchange row
-// (R1*R2 complex), tw[k] = exp(-2 pi i k/N), g[k] the kernel at frequency index k.
+// (R1*R2 complex). The thread's twiddles tw[k] = exp(-2 pi i jk/N) and kernel values
+// g[u*R2 + f] = g(jU + u + R1 f) are held in registers.
 template <int R1, int R2>
 __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const double2* tw, const double* g) {
-    constexpr int N = R1 * R2;
     constexpr int U = R1 / R2;
     dft<R1, false>(z);
 #pragma unroll
-    for (int k = 1; k < R1; k++) z[k] = c_mul(z[k], tw[(j * k) % N]);
+    for (int k = 1; k < R1; k++) z[k] = c_mul(z[k], tw[k]);
     __syncthreads();  // previous readers of the exchange rows are done
 #pragma unroll
     for (int k = 0; k < R1; k++) lx[k * R2 + j] = z[k];
@@ -109,13 +109,11 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
         for (int jj = 0; jj < R2; jj++) y[u * R2 + jj] = lx[(j * U + u) * R2 + jj];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        const int kp = j * U + u;
         dft<R2, false>(y + u * R2);
 #pragma unroll
         for (int f = 0; f < R2; f++) {
-            const double gg = g[kp + R1 * f];
-            y[u * R2 + f].x *= gg;
-            y[u * R2 + f].y *= gg;
+            y[u * R2 + f].x *= g[u * R2 + f];
+            y[u * R2 + f].y *= g[u * R2 + f];
         }
         dft<R2, true>(y + u * R2);
     }
@@ -128,7 +126,7 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 #pragma unroll
     for (int k = 0; k < R1; k++) z[k] = lx[k * R2 + j];
 #pragma unroll
-    for (int k = 1; k < R1; k++) z[k] = c_mulc(z[k], tw[(j * k) % N]);
+    for (int k = 1; k < R1; k++) z[k] = c_mulc(z[k], tw[k]);
     dft<R1, true>(z);
 }
 
@@ -138,53 +136,62 @@ template <int N, int R1, int R2>
 __global__ __launch_bounds__((N / 2) * R2) void k_slab(SlabArgs a) {
     static_assert(R1 * R2 == N && R1 % R2 == 0, "unsupported factorisation");
     constexpr int H = N / 2;           // complex lines per plane
-    constexpr int NT = H * R2;         // threads
+    constexpr int U = R1 / R2;
     constexpr int XP = N + 1;          // exchange row pitch (complex)
     constexpr int RP = N + 1;          // real plane row pitch (doubles)
     constexpr int XBYTES = H * XP * 16;
     constexpr int RBYTES = N * RP * 8;
     constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
     __shared__ __align__(16) unsigned char smem[BYTES];
-    __shared__ double2 stw[N];
-    __shared__ double sga[N], sgb[N];
     double2* xbuf = reinterpret_cast<double2*>(smem);
     double* rp = reinterpret_cast<double*>(smem);
 
     const int t = threadIdx.x;
-    for (int i = t; i < N; i += NT) {
-        stw[i] = a.tw[i];
-        sga[i] = a.ga[i];
-        sgb[i] = a.gb[i];
-    }
     const int f = blockIdx.y;
     const long pbase = (long)blockIdx.x * a.plane_stride;  // plane origin within a field
     const long base = (long)f * a.M + pbase;
+    const int c = t % H, j = t / H;    // layout A: a-lines, column pair (c, c + H), a = j + R2 k
+    const int r = t / R2, jb = t % R2; // layout B: z-lines, row pair (r, r + H), b = jb + R2 k
 
-    // layout A: a-lines, column pair (c, c + H), thread j holds a = j + R2 k
-    const int c = t % H, j = t / H;
-    double2 z[R1];
+    // everything this thread reads is issued up front: the partial pair, the Boltzmann
+    // factor, and its twiddles and kernel values (layout A and layout B)
+    double2 z[R1], hz[R1];
+    const double* hsel = f ? a.hqd : a.hq;
 #pragma unroll
     for (int k = 0; k < R1; k++) {
-        const long p = base + (long)(j + R2 * k) * a.Sa + c;
-        z[k] = make_double2(a.Tin[p], a.Tin[p + H]);
+        const long p = pbase + (long)(j + R2 * k) * a.Sa + c;
+        z[k] = make_double2(a.Tin[(long)f * a.M + p], a.Tin[(long)f * a.M + p + H]);
+        if (a.finish) hz[k] = make_double2(hsel[p], hsel[p + H]);
     }
-    __syncthreads();  // tables
+    double2 twA[R1], twB[R1];
+    double gA[R1], gB[R1];
+#pragma unroll
+    for (int k = 0; k < R1; k++) {
+        twA[k] = a.tw[j * k];
+        twB[k] = a.tw[jb * k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int ff = 0; ff < R2; ff++) {
+            gA[u * R2 + ff] = a.ga[j * U + u + R1 * ff];
+            gB[u * R2 + ff] = a.gb[jb * U + u + R1 * ff];
+        }
 
     if (a.finish) {
-        conv_line<R1, R2>(z, xbuf + c * XP, j, stw, sga);
-        const double* h = f ? a.hqd : a.hq;
+        conv_line<R1, R2>(z, xbuf + c * XP, j, twA, gA);
 #pragma unroll
         for (int k = 0; k < R1; k++) {
-            const long p = pbase + (long)(j + R2 * k) * a.Sa + c;
-            z[k].x *= h[p];
-            z[k].y *= h[p + H];
-            a.qout[(long)f * a.M + p] = z[k].x;
-            a.qout[(long)f * a.M + p + H] = z[k].y;
+            const long p = (long)f * a.M + pbase + (long)(j + R2 * k) * a.Sa + c;
+            z[k].x *= hz[k].x;
+            z[k].y *= hz[k].y;
+            a.qout[p] = z[k].x;
+            a.qout[p + H] = z[k].y;
         }
     }
     if (!a.cont) return;
 
-    conv_line<R1, R2>(z, xbuf + c * XP, j, stw, sga);
+    conv_line<R1, R2>(z, xbuf + c * XP, j, twA, gA);
     // re-pair the real plane: rows (r, r + H) become the complex z-lines
     __syncthreads();
 #pragma unroll
@@ -194,14 +201,13 @@ __global__ __launch_bounds__((N / 2) * R2) void k_slab(SlabArgs a) {
         rp[ai * RP + c + H] = z[k].y;
     }
     __syncthreads();
-    const int r = t / R2, jb = t % R2;
     double2 w[R1];
 #pragma unroll
     for (int k = 0; k < R1; k++) {
         const int b = jb + R2 * k;
         w[k] = make_double2(rp[r * RP + b], rp[(r + H) * RP + b]);
     }
-    conv_line<R1, R2>(w, xbuf + r * XP, jb, stw, sgb);
+    conv_line<R1, R2>(w, xbuf + r * XP, jb, twB, gB);
 #pragma unroll
     for (int k = 0; k < R1; k++) {
         const long p = base + (long)r * a.Sa + jb + R2 * k;
