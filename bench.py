@@ -62,6 +62,19 @@ def algorithmic_bytes(M, N, nAM_list, with_langevin=True):
     return total
 
 
+def measured_traffic(mesh):
+    """HBM bytes per contour-step launch from the committed FETCH_SIZE/WRITE_SIZE
+    summary (two rocprofv3 --pmc runs, scaled by a copy of known size); None when no
+    summary exists for this mesh."""
+    path = os.path.join(ROOT, "profiles", f"k_slab_{mesh}cubed_N90_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(w_state, params, m, Lb, N, NA, seconds_hint=30.0):
     """The oracle (CPU restatement, OpenMP over FFT lines) timed on this host for one
     Langevin step of the same workload, started from the GPU's state after warm-up."""
@@ -192,7 +205,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": measured_traffic(args.mesh),
             "kernel": "contour step (48*M algorithmic bytes), HIP events on the context stream",
             "contour_step_us": cs_ms * 1e3,
         },
