@@ -105,6 +105,82 @@ def cpu_baseline(w_state, params, m, Lb, N, NA, seconds_hint=30.0):
     }
 
 
+def batched_leg(steps, warmup, L_mod, torch, quick=False):
+    """Replica set of the batched workload: 64 members (24 x 32^3, 24 x 48^3, 16 x 64^3),
+    N=90, NA uniform in [30,60], XbN uniform in [12,20], box edge uniform in [3.5,5.0],
+    fields N(0, 0.1^2), member r seeded 1000+r (numpy) for its fields and parameters and
+    1000+r (mt19937_64) for its noise. Times `steps` whole Langevin steps of the set, then
+    the same steps of every member run alone, one after the other, from the same states."""
+    meshes = [32] * 24 + [48] * 24 + [64] * 16
+    if quick:
+        meshes = [32] * 3 + [48] * 3 + [64] * 2
+    members = []
+    for r, n in enumerate(meshes):
+        g = np.random.default_rng(1000 + r)
+        NA = int(g.integers(30, 61))
+        XbN = float(g.uniform(12.0, 20.0))
+        Lb = float(g.uniform(3.5, 5.0))
+        M = n ** 3
+        w = np.concatenate([g.normal(0.0, 0.1, M), g.normal(0.0, 0.1, M)])
+        sigma = math.sqrt(2.0 * M * 0.5 / (50.0 * Lb ** 3))
+        members.append(dict(n=n, NA=NA, XbN=XbN, L=Lb, M=M, w=w, sigma=sigma, seed=1000 + r))
+    ctxs = []
+    for mb in members:
+        c = L_mod.Context(90, mb["NA"], (mb["n"],) * 3, (mb["L"],) * 3)
+        c.set_fields(mb["w"])
+        rng = L_mod.Rng(mb["seed"], mb["sigma"])
+        c.langevin_init(rng.normals(mb["M"]))
+        mb["rng"] = rng
+        ctxs.append(c)
+    XbN = [mb["XbN"] for mb in members]
+    nsteps = warmup + steps
+    etas = [[torch.from_numpy(mb["rng"].normals(mb["M"])).cuda() for mb in members] for _ in range(nsteps)]
+    torch.cuda.synchronize()
+    B = L_mod.Batch(ctxs)
+    t0 = time.perf_counter()
+    k0, _ = B.anderson_mix(200, 1e-4)
+    log(f"batched set: initial mix {time.perf_counter() - t0:.2f} s, k in [{k0.min()}, {k0.max()}]")
+    for s in range(warmup):
+        B.full_step_dev([e.data_ptr() for e in etas[s]], XbN, 0.5)
+    # snapshot the states so that the sequential leg starts from the same point
+    snap = [(c.get_fields(4), None) for c in ctxs]
+    B.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kb = []
+    for s in range(warmup, nsteps):
+        _, k = B.full_step_dev([e.data_ptr() for e in etas[s]], XbN, 0.5)
+        kb.append(k.copy())
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    ms_chain, n_pass, ms_pe, n_pe = B.profile_read()
+    B.close()
+    # bytes moved per the contour-step accounting (48 M per member step)
+    cs_bytes = sum(48.0 * mb["M"] * (90 - 1) * (int(k) - 1 + 1) for ks in kb for mb, k in zip(members, ks))
+    # the same steps, member by member
+    for c, (w, _) in zip(ctxs, snap):
+        c.set_fields(w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r, c in enumerate(ctxs):
+        for s in range(warmup, nsteps):
+            c.full_step_dev(etas[s][r].data_ptr(), XbN[r], 0.5)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    n_members = len(members)
+    return {
+        "members": n_members,
+        "meshes": {"32^3": meshes.count(32), "48^3": meshes.count(48), "64^3": meshes.count(64)},
+        "steps": steps,
+        "member_steps_per_s": n_members * steps / t_batch,
+        "sequential_member_steps_per_s": n_members * steps / t_seq,
+        "speedup_vs_sequential": t_seq / t_batch,
+        "mean_n_AM": float(np.mean([np.mean(k) - 1 for k in kb])),
+        "contour_chain_algorithmic_GBps": cs_bytes / (ms_chain * 1e-3) / 1e9 if ms_chain > 0 else None,
+        "chain_frac_of_8TBps": cs_bytes / (ms_chain * 1e-3) / 8e12 if ms_chain > 0 else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +188,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mesh", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batched-steps", type=int, default=3, help="timed steps of the replica-set leg (0 skips it)")
+    ap.add_argument("--batched-quick", action="store_true", help="8 members instead of 64 (smoke runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -178,6 +256,11 @@ def main():
         except Exception as e:  # the baseline is reported, not required
             log(f"cpu baseline failed: {e!r}")
 
+    batched = None
+    if args.batched_steps > 0:
+        ctx.close()
+        batched = batched_leg(args.batched_steps, 1, L, torch, quick=args.batched_quick)
+
     out = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -214,6 +297,7 @@ def main():
         "contour_steps_M_per_s": cs_per_s,
         "propagator_eval_ms": ms_pe / max(n_pe, 1),
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

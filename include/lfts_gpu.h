@@ -108,11 +108,37 @@ int lfts_counters(lfts_ctx* ctx, long long* n_prop_evals, long long* n_contour_s
 /* Live timing with HIP events recorded on the context stream (no reference counterpart;
  * the reference only prints whole-second wall times, src/fts_cpu.cc:44-57). While enabled,
  * every propagator evaluation records the duration of its contour-step chain (the N-1
- * contour steps, back to back) and of the whole evaluation. lfts_profile_read returns
- * the sums (ms) and counts since the last enable. */
+ * contour-step launches, back to back) and of the whole evaluation. lfts_profile_read
+ * returns the sums (ms) and counts (contour-step launches, evaluations) since the last
+ * enable. For a replica set, one contour-step launch advances every member of one mesh
+ * size, and evaluations are counted per member. */
 int lfts_profile(lfts_ctx* ctx, int enable);
 int lfts_profile_read(lfts_ctx* ctx, double* ms_chain, long long* n_contour_steps, double* ms_pe,
                       long long* n_prop_evals);
+
+/* Replica sets (no reference counterpart: the reference runs one system per process,
+ * src/fts_cpu.cc:40). A set advances many independent contexts together: one launch per
+ * contour step for every mesh size present, one host synchronisation per Anderson
+ * iteration for the whole set. Every entry point has the meaning of the single-context
+ * call above applied to each member, and gives each member the same result as that call
+ * (the kernels and reduction orders are the same). Arrays are indexed by member; members
+ * keep their own Anderson iteration counts, and a member whose loop condition ends leaves
+ * the active set while the others continue. A context belongs to at most one set in use
+ * at a time; the set must be destroyed before its members. */
+typedef struct lfts_batch lfts_batch;
+int lfts_batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out);
+int lfts_batch_destroy(lfts_batch* b);
+const char* lfts_batch_last_error(const lfts_batch* b);
+int lfts_batch_get_stream(lfts_batch* b, void** stream);
+int lfts_batch_calc_concs(lfts_batch* b, double* lnQ);
+int lfts_batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, double* err_ret);
+int lfts_batch_langevin_step_dev(lfts_batch* b, const double* const* eta_new_dev, const double* XbN,
+                                 const double* dt);
+int lfts_batch_full_step_dev(lfts_batch* b, const double* const* eta_new_dev, const double* XbN, const double* dt,
+                             int maxIter, double errTol, double* lnQ, int* k_ret);
+int lfts_batch_profile(lfts_batch* b, int enable);
+int lfts_batch_profile_read(lfts_batch* b, double* ms_chain, long long* n_passes, double* ms_pe,
+                            long long* n_prop_evals);
 
 /* Host random stream of the reference: std::mt19937_64 seeded by the caller (the
  * reference seeds 123456789, src/lfts_simulation.cc:77) feeding one persistent

@@ -67,6 +67,17 @@ SIGNATURES = {
     "lfts_counters": (ctypes.c_int, [_vp, _llp, _llp]),
     "lfts_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lfts_profile_read": (ctypes.c_int, [_vp, _dp, _llp, _dp, _llp]),
+    "lfts_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
+    "lfts_batch_destroy": (ctypes.c_int, [_vp]),
+    "lfts_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "lfts_batch_get_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "lfts_batch_calc_concs": (ctypes.c_int, [_vp, _dp]),
+    "lfts_batch_anderson_mix": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _ip, _dp]),
+    "lfts_batch_langevin_step_dev": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _dp, _dp]),
+    "lfts_batch_full_step_dev": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                                _dp, _ip]),
+    "lfts_batch_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lfts_batch_profile_read": (ctypes.c_int, [_vp, _dp, _llp, _dp, _llp]),
     "lfts_rng_create": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_double, ctypes.POINTER(_vp)]),
     "lfts_rng_normals": (ctypes.c_int, [_vp, _dp, ctypes.c_longlong]),
     "lfts_rng_uniforms": (ctypes.c_int, [_vp, _dp, ctypes.c_longlong]),
@@ -228,6 +239,76 @@ class Context:
         b = ctypes.c_longlong()
         self._chk(lib().lfts_counters(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+class Batch:
+    """A replica set: many contexts advanced together (one launch per contour step per
+    mesh size). Keep the member contexts alive while the set is in use."""
+
+    def __init__(self, ctxs):
+        self.ctxs = list(ctxs)
+        self.n = len(self.ctxs)
+        arr = (_vp * self.n)(*[c.h for c in self.ctxs])
+        h = _vp()
+        rc = lib().lfts_batch_create(arr, self.n, ctypes.byref(h))
+        if rc != LFTS_OK:
+            raise LftsError(rc, "lfts_batch_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lfts_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc: int, allow_enum: bool = False):
+        if rc != LFTS_OK and not (allow_enum and rc == LFTS_ENUM):
+            raise LftsError(rc, lib().lfts_batch_last_error(self.h).decode())
+
+    def stream(self) -> int:
+        p = _vp()
+        self._chk(lib().lfts_batch_get_stream(self.h, ctypes.byref(p)))
+        return p.value or 0
+
+    def calc_concs(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        self._chk(lib().lfts_batch_calc_concs(self.h, _d(out)))
+        return out
+
+    def anderson_mix(self, maxIter: int = 200, tol: float = 1e-4):
+        k = np.zeros(self.n, dtype=np.int32)
+        err = np.zeros(self.n)
+        self._chk(lib().lfts_batch_anderson_mix(self.h, int(maxIter), float(tol), _i(k), _d(err)))
+        return k, err
+
+    def _ptrs(self, eta_ptrs):
+        return (_vp * self.n)(*[ctypes.c_void_p(int(p)) for p in eta_ptrs])
+
+    def langevin_step_dev(self, eta_ptrs, XbN, dt):
+        XbN = np.ascontiguousarray(np.broadcast_to(np.asarray(XbN, dtype=np.float64), (self.n,)))
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.n,)))
+        self._chk(lib().lfts_batch_langevin_step_dev(self.h, self._ptrs(eta_ptrs), _d(XbN), _d(dt)))
+
+    def full_step_dev(self, eta_ptrs, XbN, dt, maxIter: int = 200, tol: float = 1e-4):
+        XbN = np.ascontiguousarray(np.broadcast_to(np.asarray(XbN, dtype=np.float64), (self.n,)))
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.n,)))
+        lnq = np.zeros(self.n)
+        k = np.zeros(self.n, dtype=np.int32)
+        self._chk(lib().lfts_batch_full_step_dev(self.h, self._ptrs(eta_ptrs), _d(XbN), _d(dt), int(maxIter),
+                                                 float(tol), _d(lnq), _i(k)))
+        return lnq, k
+
+    def profile(self, enable: bool = True):
+        self._chk(lib().lfts_batch_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        a, c = ctypes.c_double(), ctypes.c_double()
+        b, d = ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(lib().lfts_batch_profile_read(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                                ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
 
 
 class Rng:

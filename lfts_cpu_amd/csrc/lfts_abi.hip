@@ -1,6 +1,10 @@
-// C-ABI layer of the L-FTS Langevin-step library: per-system context, plans, device
-// memory, the host control of Anderson mixing (its small LU solve stays on the host),
-// the reference's host random stream, and the host-side S(k) binning.
+// C-ABI layer of the L-FTS Langevin-step library.
+//
+// A context (lfts_ctx) owns one system's device state and plans. All work runs on
+// replica sets (lfts_batch): a context's own calls use its private set of one, and
+// lfts_batch_* drives many contexts together, one launch per contour step for every
+// mesh size present. The Anderson least-squares solve (<= 15 x 15 LU), the random
+// stream of the reference and the S(k) binning stay on the host.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,62 +21,88 @@
 #include "lfts_gpu.h"
 #include "lfts_internal.h"
 
-using lfts::AmCoef;
+using lfts::Coef;
 using lfts::LineArgs;
+using lfts::Rep;
 
 namespace {
 
 struct AxisPlan {
     int n = 1, nst = 0;
     int radix[lfts::MAXST] = {0};
-    double2* tw = nullptr;
-    double* gperm = nullptr;
-    int* perm = nullptr;
+    double2* tw = nullptr;    // exp(-2 pi i k/n)
+    double* gperm = nullptr;  // kernel at in-place DIF positions (line path)
+    int* perm = nullptr;      // DIF position of frequency k
+    double* gnat = nullptr;   // kernel in natural frequency order (slab path)
     int T = 1, pitch = 2;
-    double* gnat = nullptr;  // kernel in natural frequency order (g(k)/n), for the fused path
 };
 
 }  // namespace
+
+struct lfts_batch;
 
 struct lfts_ctx {
     lfts_desc d{};
     long M = 0, Mk = 0;
     int N = 0, NA = 0, NB = 0, nhMax = 10;
-    hipStream_t st = nullptr;
     AxisPlan ax[3];
-    double* w = nullptr;      // 4M: W-, w+, phi-, phi+
-    double* h = nullptr;      // 2M: hA, hB (= pair 1)
-    double* q = nullptr;      // (N-1) pairs for beads 2..N
-    double* tmp = nullptr;    // 2M
-    double* tmp2 = nullptr;   // 2M (second partial-convolution buffer of the fused path)
-    bool fused = false;       // one launch per contour step (slab passes)
+    bool fused = false;       // cubic 32/48/64 mesh: one slab launch per contour step
+    double* w = nullptr;      // 4M
+    double* h = nullptr;      // 2M (pair 1)
+    double* q = nullptr;      // (N-1) pairs
+    double* T0 = nullptr;     // 2M partial pair (slab path) / line-path scratch
+    double* T1 = nullptr;     // 2M partial pair (slab path)
     double* Dh = nullptr;     // (nhMax+1) M
     double* wh = nullptr;     // (nhMax+1) M
-    double* noise = nullptr;  // 2M: eta_prev | staging for eta_new
-    int eta_prev = 0;         // which half holds eta_prev
+    double* noise = nullptr;  // 2M: eta_prev | staging for a host eta_new
     double* S = nullptr;      // Mk
-    double2* zk = nullptr;    // M, S(k) transform scratch
-    double* part = nullptr;   // nred * PSTRIDE
-    double* scal = nullptr;   // 64
-    double* pinned = nullptr; // host staging, 64 doubles
-    double* test = nullptr;   // 4M scratch for lfts_step_fwd
+    double2* zk = nullptr;    // M, S(k) transform scratch (allocated on first sample)
+    double* test = nullptr;   // 8M, lfts_step_fwd scratch (allocated on first use)
     int nsamples = 0;
-    int nblk = 1, nred = 1;
     long long n_pe = 0, n_cs = 0;
-    // live event timing (lfts_profile)
-    bool prof = false;
-    std::vector<hipEvent_t> ev_pool;  // free events
-    struct EvQuad { hipEvent_t pe0, ch0, ch1, pe1; int nsteps; };
-    std::vector<EvQuad> ev_pending;
-    double prof_ms_chain = 0.0, prof_ms_pe = 0.0;
-    long long prof_n_cs = 0, prof_n_pe = 0;
-    std::vector<double> DD;   // (nhMax+1)^2
-    std::vector<double> U, V, C;
+    std::vector<double> DD, U, V, C;  // Anderson host state (src/anderson.cc:8-21)
+    lfts_batch* self = nullptr;       // set of one for the single-context entry points
     std::string err;
 };
 
-static int fail(lfts_ctx* c, int code, const std::string& msg) {
+struct lfts_batch {
+    std::vector<lfts_ctx*> ctx;
+    hipStream_t st = nullptr;
+    std::vector<Rep> reps;      // descriptors of all members (id = member index)
+    Rep* d_ract = nullptr;      // descriptors of the active members, in launch order
+    Rep* h_ract = nullptr;      // pinned staging of d_ract
+    int nb_max = 1;             // largest workgroup count among the active members
+    Coef* d_coef = nullptr;
+    Coef* h_coef = nullptr;     // pinned
+    double* d_part = nullptr;
+    double* d_scal = nullptr;
+    double* h_scal = nullptr;   // pinned
+    // the active set as launched: slab replicas sorted by edge, then line-path ones
+    std::vector<int> act;
+    struct Group { int edge, off, cnt; const double2* tw; };
+    std::vector<Group> groups;
+    std::vector<int> line_reps;
+    // live event timing
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct EvQuad { hipEvent_t pe0, ch0, ch1, pe1; long long npasses; int nrep; };
+    std::vector<EvQuad> ev_pending;
+    double prof_ms_chain = 0.0, prof_ms_pe = 0.0;
+    long long prof_n_pass = 0, prof_n_pe = 0;
+    std::string err;
+};
+
+namespace {
+
+int fail(lfts_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
+    return code;
+}
+int bfail(lfts_batch* b, int code, const std::string& msg) {
+    if (b) {
+        b->err = msg;
+        for (lfts_ctx* c : b->ctx) c->err = msg;
+    }
     return code;
 }
 
@@ -82,8 +112,14 @@ static int fail(lfts_ctx* c, int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                                 \
             return fail((ctx), LFTS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+#define BHIPCHK(b, expr)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return bfail((b), LFTS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
 
-static bool factorize(int n, std::vector<int>& r) {
+bool factorize(int n, std::vector<int>& r) {
     r.clear();
     while (n % 4 == 0) { r.push_back(4); n /= 4; }
     while (n % 2 == 0) { r.push_back(2); n /= 2; }
@@ -95,13 +131,13 @@ static bool factorize(int n, std::vector<int>& r) {
 // Position of frequency k after the in-place decimation-in-frequency pass: a stage of
 // radix r on a block of size B leaves the sub-DFT (k mod r) in sub-block k mod r, so
 // pos_B(k) = (B/r)(k mod r) + pos_{B/r}(k div r).
-static int dif_pos(int k, const std::vector<int>& rad, size_t s, int B) {
+int dif_pos(int k, const std::vector<int>& rad, size_t s, int B) {
     if (s == rad.size()) return 0;
     const int r = rad[s], S = B / r;
     return S * (k % r) + dif_pos(k / r, rad, s + 1, S);
 }
 
-static int build_axis(lfts_ctx* c, AxisPlan& p, int n, double L, int N) {
+int build_axis(lfts_ctx* c, AxisPlan& p, int n, double L, int N) {
     std::vector<int> rad;
     if (!factorize(n, rad) || (int)rad.size() > lfts::MAXST)
         return fail(c, LFTS_EINVAL, "mesh size " + std::to_string(n) + " has a prime factor above 7");
@@ -115,17 +151,15 @@ static int build_axis(lfts_ctx* c, AxisPlan& p, int n, double L, int N) {
     }
     // Per-axis factor of g(k) = exp(-K^2/(6N))/M, K = 2 pi |k|/L (src/step.cc:84-85),
     // with the reference's frequency convention k in [-(n-1)/2, n/2] (src/step.cc:73-83).
-    std::vector<double> gperm(n);
+    std::vector<double> gperm(n), gnat(n);
     std::vector<int> perm(n);
     for (int k = 0; k < n; k++) {
         const int f = (k <= n / 2) ? k : k - n;
         const double K = 2.0 * M_PI * (double)f / L;
-        const int pos = dif_pos(k, rad, 0, n);
-        perm[k] = pos;
-        gperm[pos] = std::exp(-K * K / (6.0 * N)) / (double)n;
+        gnat[k] = std::exp(-K * K / (6.0 * N)) / (double)n;
+        perm[k] = dif_pos(k, rad, 0, n);
+        gperm[perm[k]] = gnat[k];
     }
-    std::vector<double> gnat(n);
-    for (int k = 0; k < n; k++) gnat[k] = gperm[perm[k]];
     HIPCHK(c, hipMalloc(&p.tw, sizeof(double2) * n));
     HIPCHK(c, hipMalloc(&p.gperm, sizeof(double) * n));
     HIPCHK(c, hipMalloc(&p.perm, sizeof(int) * n));
@@ -139,7 +173,29 @@ static int build_axis(lfts_ctx* c, AxisPlan& p, int n, double L, int N) {
     return LFTS_OK;
 }
 
-static LineArgs line_args(const lfts_ctx* c, int axis, int mode) {
+Rep make_rep(const lfts_ctx* c) {
+    Rep r;
+    std::memset(&r, 0, sizeof r);
+    r.w = c->w;
+    r.h = c->h;
+    r.q = c->q;
+    r.T0 = c->T0;
+    r.T1 = c->T1;
+    for (int d = 0; d < 3; d++) r.g[d] = c->ax[d].gnat;
+    r.Dh = c->Dh;
+    r.wh = c->wh;
+    r.eta_prev = c->noise;
+    r.M = c->M;
+    r.N = c->N;
+    r.NA = c->NA;
+    r.NB = c->NB;
+    r.n = c->fused ? c->d.m[0] : 0;
+    r.id = 0;
+    r.nb = lfts::blocks_for(c->M);
+    return r;
+}
+
+LineArgs line_args(const lfts_ctx* c, int axis, int mode) {
     const AxisPlan& p = c->ax[axis];
     LineArgs a;
     std::memset(&a, 0, sizeof a);
@@ -161,145 +217,32 @@ static LineArgs line_args(const lfts_ctx* c, int axis, int mode) {
     return a;
 }
 
-static double* pair_ptr(lfts_ctx* c, int i) { return (i == 1) ? c->h : c->q + 2L * (i - 2) * c->M; }
+double* pair_ptr(lfts_ctx* c, int i) { return (i == 1) ? c->h : c->q + 2L * (i - 2) * c->M; }
 
-// src/step.cc:35-65: out = h_sel * conv_g(in) for the pair (q_i, q+_{N+1-i}).
-static int contour_step(lfts_ctx* c, const double* in, double* out, const double* h, int i) {
+// Line path, src/step.cc:35-65: out = h_sel * conv_g(in) for the pair (q_i, q+_{N+1-i}),
+// as three launches (z, y, then x with the Boltzmann factor); `tmp` holds 2M doubles.
+int line_contour_step(lfts_ctx* c, const double* in, double* out, const double* h, int i, double* tmp,
+                      hipStream_t st) {
     const long M = c->M;
     LineArgs z = line_args(c, 2, lfts::LINE_CONV);
-    z.inA = in; z.inB = in + M; z.outA = c->tmp; z.outB = c->tmp + M;
-    HIPCHK(c, lfts::launch_line(z, c->st));
+    z.inA = in; z.inB = in + M; z.outA = tmp; z.outB = tmp + M;
+    HIPCHK(c, lfts::launch_line(z, st));
     LineArgs y = line_args(c, 1, lfts::LINE_CONV);
-    y.inA = c->tmp; y.inB = c->tmp + M; y.outA = c->tmp; y.outB = c->tmp + M;
-    HIPCHK(c, lfts::launch_line(y, c->st));
+    y.inA = tmp; y.inB = tmp + M; y.outA = tmp; y.outB = tmp + M;
+    HIPCHK(c, lfts::launch_line(y, st));
     LineArgs x = line_args(c, 0, lfts::LINE_CONV);
-    x.inA = c->tmp; x.inB = c->tmp + M; x.outA = out; x.outB = out + M;
+    x.inA = tmp; x.inB = tmp + M; x.outA = out; x.outB = out + M;
     x.hA = (i < c->NA) ? h : h + M;       // q_{i+1}   (src/step.cc:53-57)
     x.hB = (i < c->NB) ? h + M : h;       // q+_{N-i} (src/step.cc:60-64)
-    HIPCHK(c, lfts::launch_line(x, c->st));
-    c->n_cs++;
-    return LFTS_OK;
-}
-
-static int get_event(lfts_ctx* c, hipEvent_t* e) {
-    if (!c->ev_pool.empty()) {
-        *e = c->ev_pool.back();
-        c->ev_pool.pop_back();
-        return LFTS_OK;
-    }
-    HIPCHK(c, hipEventCreate(e));
-    return LFTS_OK;
-}
-
-// Fold completed event quads into the profile sums (the stream has been synchronised).
-static int resolve_events(lfts_ctx* c) {
-    for (auto& q : c->ev_pending) {
-        float a = 0.f, b = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&a, q.ch0, q.ch1));
-        HIPCHK(c, hipEventElapsedTime(&b, q.pe0, q.pe1));
-        c->prof_ms_chain += a;
-        c->prof_ms_pe += b;
-        c->prof_n_cs += q.nsteps;
-        c->prof_n_pe += 1;
-        for (hipEvent_t e : {q.pe0, q.ch0, q.ch1, q.pe1}) c->ev_pool.push_back(e);
-    }
-    c->ev_pending.clear();
-    return LFTS_OK;
-}
-
-// The N-1 contour steps of src/diblock.cc:43 as alternating slab passes, one launch
-// per step: a prologue x-slab pass convolves q_1 along y and z, then pass i (i = 1..N-1)
-// finishes step i (y-slab for odd i, x-slab for even i), stores q_{i+1} and starts step
-// i+1.
-static int fused_chain(lfts_ctx* c) {
-    const long M = c->M;
-    const int n = c->d.m[0];
-    const long m1 = c->d.m[1], m2 = c->d.m[2];
-    double* T[2] = {c->tmp, c->tmp2};
-    auto setup = [&](lfts::SlabArgs& a, bool xslab) {
-        std::memset(&a, 0, sizeof a);
-        a.M = M;
-        a.tw = c->ax[2].tw;
-        a.gb = c->ax[2].gnat;
-        if (xslab) {  // plane x0: outer axis y
-            a.plane_stride = m1 * m2;
-            a.Sa = m2;
-            a.ga = c->ax[1].gnat;
-        } else {      // plane y0: outer axis x
-            a.plane_stride = m2;
-            a.Sa = m1 * m2;
-            a.ga = c->ax[0].gnat;
-        }
-    };
-    lfts::SlabArgs a;
-    setup(a, true);
-    a.Tin = c->h;  // pair 1 = [q_1 | q+_N] = [hA | hB]
-    a.Tout = T[0];
-    a.finish = 0;
-    a.cont = 1;
-    HIPCHK(c, lfts::launch_slab(a, n, c->d.m[0], c->st));
-    for (int i = 1; i < c->N; i++) {
-        const bool xslab = (i % 2 == 0);
-        setup(a, xslab);
-        a.Tin = T[(i - 1) & 1];
-        a.Tout = T[i & 1];
-        a.qout = pair_ptr(c, i + 1);
-        a.hq = (i < c->NA) ? c->h : c->h + M;      // q_{i+1}   (src/step.cc:53-57)
-        a.hqd = (i < c->NB) ? c->h + M : c->h;     // q+_{N-i} (src/step.cc:60-64)
-        a.finish = 1;
-        a.cont = (i < c->N - 1) ? 1 : 0;
-        HIPCHK(c, lfts::launch_slab(a, n, xslab ? c->d.m[0] : c->d.m[1], c->st));
-        c->n_cs++;
-    }
-    return LFTS_OK;
-}
-
-// src/diblock.cc:30-60, without the host read of Q; scal[0] = Q.
-static int calc_concs_async(lfts_ctx* c) {
-    const long M = c->M;
-    lfts_ctx::EvQuad ev{};
-    if (c->prof) {
-        for (hipEvent_t* e : {&ev.pe0, &ev.ch0, &ev.ch1, &ev.pe1})
-            if (int rc = get_event(c, e)) return rc;
-        HIPCHK(c, hipEventRecord(ev.pe0, c->st));
-    }
-    HIPCHK(c, lfts::launch_prepare_h(c->w, c->h, M, c->N, c->nblk, c->st));
-    if (c->prof) HIPCHK(c, hipEventRecord(ev.ch0, c->st));
-    if (c->fused) {
-        if (int rc = fused_chain(c)) return rc;
-    } else {
-        for (int i = 1; i < c->N; i++) {
-            int rc = contour_step(c, pair_ptr(c, i), pair_ptr(c, i + 1), c->h, i);
-            if (rc) return rc;
-        }
-    }
-    if (c->prof) HIPCHK(c, hipEventRecord(ev.ch1, c->st));
-    HIPCHK(c, lfts::launch_sum_partial(pair_ptr(c, c->N), M, c->part, c->nred, c->st));
-    HIPCHK(c, lfts::launch_reduce_final(c->part, c->nred, 1, (double)M, c->scal, c->st));
-    HIPCHK(c, lfts::launch_phi(c->h, c->q, c->w, c->scal, M, c->N, c->NA, c->nblk, c->st));
-    if (c->prof) {
-        HIPCHK(c, hipEventRecord(ev.pe1, c->st));
-        ev.nsteps = c->N - 1;
-        c->ev_pending.push_back(ev);
-    }
-    c->n_pe++;
-    return LFTS_OK;
-}
-
-static int read_scalars(lfts_ctx* c, int n, double* out) {
-    HIPCHK(c, hipMemcpyAsync(c->pinned, c->scal, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    std::memcpy(out, c->pinned, sizeof(double) * n);
-    if (!c->ev_pending.empty()) return resolve_events(c);
+    HIPCHK(c, lfts::launch_line(x, st));
     return LFTS_OK;
 }
 
 // LU with partial pivoting and solve (the GSL LU_decomp/LU_solve of src/anderson.cc:118-126).
-static bool lu_solve(int n, const double* A_in, const double* b, double* x) {
+void lu_solve(int n, const double* A_in, const double* b, double* x) {
     std::vector<double> A(A_in, A_in + (size_t)n * n);
     std::vector<int> p(n);
     std::iota(p.begin(), p.end(), 0);
-    bool singular = false;
     for (int j = 0; j < n; j++) {
         int ip = j;
         double amax = std::fabs(A[(size_t)j * n + j]);
@@ -312,7 +255,7 @@ static bool lu_solve(int n, const double* A_in, const double* b, double* x) {
             std::swap(p[j], p[ip]);
         }
         const double ajj = A[(size_t)j * n + j];
-        if (ajj == 0.0) { singular = true; continue; }
+        if (ajj == 0.0) continue;
         for (int i = j + 1; i < n; i++) {
             const double l = A[(size_t)i * n + j] / ajj;
             A[(size_t)i * n + j] = l;
@@ -330,68 +273,326 @@ static bool lu_solve(int n, const double* A_in, const double* b, double* x) {
         for (int k = i + 1; k < n; k++) s -= A[(size_t)i * n + k] * x[k];
         x[i] = s / A[(size_t)i * n + i];
     }
-    return !singular;
 }
 
-static int check_ctx(lfts_ctx* c) { return c ? LFTS_OK : LFTS_EINVAL; }
+// ---- replica sets ----------------------------------------------------------------------
+
+int get_event(lfts_batch* b, hipEvent_t* e) {
+    if (!b->ev_pool.empty()) {
+        *e = b->ev_pool.back();
+        b->ev_pool.pop_back();
+        return LFTS_OK;
+    }
+    BHIPCHK(b, hipEventCreate(e));
+    return LFTS_OK;
+}
+
+// Fold completed event quads into the profile sums (the stream has been synchronised).
+int resolve_events(lfts_batch* b) {
+    for (auto& q : b->ev_pending) {
+        float a = 0.f, t = 0.f;
+        BHIPCHK(b, hipEventElapsedTime(&a, q.ch0, q.ch1));
+        BHIPCHK(b, hipEventElapsedTime(&t, q.pe0, q.pe1));
+        b->prof_ms_chain += a;
+        b->prof_ms_pe += t;
+        b->prof_n_pass += q.npasses;
+        b->prof_n_pe += q.nrep;
+        for (hipEvent_t e : {q.pe0, q.ch0, q.ch1, q.pe1}) b->ev_pool.push_back(e);
+    }
+    b->ev_pending.clear();
+    return LFTS_OK;
+}
+
+int sync_batch(lfts_batch* b) {
+    BHIPCHK(b, hipStreamSynchronize(b->st));
+    if (!b->ev_pending.empty()) return resolve_events(b);
+    return LFTS_OK;
+}
+
+// Make `ids` (indices into b->ctx) the active set: slab replicas sorted by edge, then
+// line-path replicas; upload their descriptors in that order. The pinned staging is
+// only rewritten after the stream has drained (callers synchronise before changing the
+// set).
+int set_active(lfts_batch* b, const std::vector<int>& ids) {
+    b->act.clear();
+    b->groups.clear();
+    b->line_reps.clear();
+    for (int edge : {64, 48, 32}) {
+        const int off = (int)b->act.size();
+        for (int id : ids)
+            if (b->ctx[id]->fused && b->ctx[id]->d.m[0] == edge) b->act.push_back(id);
+        const int cnt = (int)b->act.size() - off;
+        if (cnt) {
+            const lfts_ctx* c0 = b->ctx[b->act[off]];
+            b->groups.push_back({edge, off, cnt, c0->ax[0].tw});
+        }
+    }
+    for (int id : ids)
+        if (!b->ctx[id]->fused) {
+            b->line_reps.push_back(id);
+            b->act.push_back(id);
+        }
+    if (b->act.empty()) return LFTS_OK;
+    b->nb_max = 1;
+    for (size_t s = 0; s < b->act.size(); s++) {
+        b->h_ract[s] = b->reps[b->act[s]];
+        b->nb_max = std::max(b->nb_max, b->h_ract[s].nb);
+    }
+    BHIPCHK(b, hipMemcpyAsync(b->d_ract, b->h_ract, sizeof(Rep) * b->act.size(), hipMemcpyHostToDevice, b->st));
+    return LFTS_OK;
+}
+
+int upload_coef(lfts_batch* b) {
+    BHIPCHK(b, hipMemcpyAsync(b->d_coef, b->h_coef, sizeof(Coef) * b->ctx.size(), hipMemcpyHostToDevice, b->st));
+    return LFTS_OK;
+}
+
+int read_scal(lfts_batch* b) {
+    BHIPCHK(b, hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(double) * lfts::SSTRIDE * b->ctx.size(),
+                              hipMemcpyDeviceToHost, b->st));
+    return sync_batch(b);
+}
+
+double scal(const lfts_batch* b, int id, int k) { return b->h_scal[(size_t)id * lfts::SSTRIDE + k]; }
+
+// One propagator evaluation (src/diblock.cc:30-60) of every active replica; Q of
+// replica id lands in d_scal[id][0].
+int calc_concs_async(lfts_batch* b) {
+    const int nact = (int)b->act.size();
+    if (nact == 0) return LFTS_OK;
+    lfts_batch::EvQuad ev{};
+    if (b->prof) {
+        for (hipEvent_t* e : {&ev.pe0, &ev.ch0, &ev.ch1, &ev.pe1})
+            if (int rc = get_event(b, e)) return rc;
+        BHIPCHK(b, hipEventRecord(ev.pe0, b->st));
+    }
+    BHIPCHK(b, lfts::launch_prepare_h(b->d_ract, nact, b->nb_max, b->st));
+    if (b->prof) BHIPCHK(b, hipEventRecord(ev.ch0, b->st));
+    int Nmax = 0;
+    for (int id : b->act) Nmax = std::max(Nmax, b->ctx[id]->N);
+    long long npasses = 0;
+    // slab passes: i = 0 prologue, i = 1..N-1 contour steps (src/diblock.cc:43)
+    for (int i = 0; i < Nmax; i++)
+        for (const auto& g : b->groups) {
+            BHIPCHK(b, lfts::launch_slab(b->d_ract + g.off, g.cnt, g.edge, i, g.tw, b->st));
+            if (i > 0) npasses++;
+        }
+    for (int id : b->line_reps) {
+        lfts_ctx* c = b->ctx[id];
+        for (int i = 1; i < c->N; i++) {
+            if (int rc = line_contour_step(c, pair_ptr(c, i), pair_ptr(c, i + 1), c->h, i, c->T0, b->st))
+                return bfail(b, rc, c->err);
+            npasses++;
+        }
+    }
+    if (b->prof) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
+    BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_part, b->st));
+    BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, 1, 1, b->d_scal, 0, b->st));
+    BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, b->st));
+    if (b->prof) {
+        BHIPCHK(b, hipEventRecord(ev.pe1, b->st));
+        ev.npasses = npasses;
+        ev.nrep = nact;
+        b->ev_pending.push_back(ev);
+    }
+    for (int id : b->act) {
+        b->ctx[id]->n_pe++;
+        b->ctx[id]->n_cs += b->ctx[id]->N - 1;
+    }
+    return LFTS_OK;
+}
+
+std::vector<int> all_ids(const lfts_batch* b) {
+    std::vector<int> ids(b->ctx.size());
+    std::iota(ids.begin(), ids.end(), 0);
+    return ids;
+}
+
+int batch_calc_concs(lfts_batch* b, double* lnQ) {
+    if (int rc = set_active(b, all_ids(b))) return rc;
+    if (int rc = calc_concs_async(b)) return rc;
+    if (int rc = read_scal(b)) return rc;
+    bool finite = true;
+    for (size_t id = 0; id < b->ctx.size(); id++) {
+        const double Q = scal(b, (int)id, 0);
+        if (lnQ) lnQ[id] = std::log(Q);
+        if (!std::isfinite(Q) || Q <= 0.0) finite = false;
+    }
+    return finite ? LFTS_OK : bfail(b, LFTS_ENUM, "partition function Q is not finite and positive");
+}
+
+// src/anderson.cc:24-93, every replica with its own iteration count; replicas leave the
+// active set as their own loop condition fails, exactly as separate calls would.
+int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, double* err_ret) {
+    const int n = (int)b->ctx.size();
+    std::vector<int> k(n, 1);
+    std::vector<double> err(n, 1.0);
+    std::vector<int> active;
+    for (int id = 0; id < n; id++)
+        if (1 < maxIter && err[id] > errTol) active.push_back(id);
+    while (!active.empty()) {
+        if (int rc = set_active(b, active)) return rc;
+        if (int rc = calc_concs_async(b)) return rc;
+        int nh_max = 0;
+        for (int id : active) {
+            lfts_ctx* c = b->ctx[id];
+            Coef& co = b->h_coef[id];
+            co.nh = (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax;
+            nh_max = std::max(nh_max, co.nh);
+        }
+        if (int rc = upload_coef(b)) return rc;
+        const int nact = (int)b->act.size();
+        BHIPCHK(b, lfts::launch_am_dots(b->d_ract, nact, b->nb_max, b->d_coef, b->d_part, b->st));
+        BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, nh_max + 1, 0, b->d_scal, 0, b->st));
+        if (int rc = read_scal(b)) return rc;
+        for (int id : active) {
+            lfts_ctx* c = b->ctx[id];
+            const int H = c->nhMax + 1;
+            auto DD = [&](int i, int j) -> double& { return c->DD[(size_t)i * H + j]; };
+            Coef& co = b->h_coef[id];
+            const int nh = co.nh;
+            const double S1 = scal(b, id, 0);
+            err[id] = std::pow(S1 / c->M, 0.5);
+            for (int m = 0; m <= nh; m++) DD(0, m) = scal(b, id, m);
+            co.lambda = 1.0 - std::pow(0.9, double(k[id]));
+            if (k[id] < 2) {
+                co.nh = 0;
+            } else {
+                for (int m = 1; m <= nh; m++) {
+                    c->V[m - 1] = DD(0, 0) - DD(0, m);
+                    for (int nn = 1; nn <= m; nn++)
+                        c->U[(m - 1) * nh + nn - 1] = c->U[(nn - 1) * nh + m - 1] =
+                            DD(0, 0) - DD(0, m) - DD(0, nn) + DD(nn, m);
+                }
+                lu_solve(nh, c->U.data(), c->V.data(), c->C.data());
+                for (int nn = 0; nn < nh; nn++) co.C[nn] = c->C[nn];
+            }
+            co.slot = 1 + (k[id] - 1) % c->nhMax;
+        }
+        if (int rc = upload_coef(b)) return rc;
+        BHIPCHK(b, lfts::launch_am_update(b->d_ract, nact, b->nb_max, b->d_coef, b->st));
+        std::vector<int> still;
+        for (int id : active) {
+            lfts_ctx* c = b->ctx[id];
+            const int H = c->nhMax + 1;
+            auto DD = [&](int i, int j) -> double& { return c->DD[(size_t)i * H + j]; };
+            const int nh = (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax;
+            const int slot = 1 + (k[id] - 1) % c->nhMax;
+            DD(slot, slot) = DD(0, 0);
+            for (int m = 1; m < slot; m++) DD(m, slot) = DD(0, m);
+            for (int m = slot + 1; m <= nh; m++) DD(slot, m) = DD(0, m);
+            k[id]++;
+            if (k[id] < maxIter && err[id] > errTol) still.push_back(id);
+        }
+        // the update must finish before the pinned lists are rewritten
+        if (int rc = sync_batch(b)) return rc;
+        active.swap(still);
+    }
+    bool nonfinite = false;
+    for (int id = 0; id < n; id++) {
+        if (k_ret) k_ret[id] = k[id];
+        if (err_ret) err_ret[id] = err[id];
+        if (!std::isfinite(err[id])) nonfinite = true;
+    }
+    return nonfinite ? bfail(b, LFTS_ENUM, "Anderson residual is not finite") : LFTS_OK;
+}
+
+// src/langevin.cc:23-37 for every replica; eta_new are device pointers.
+int batch_langevin(lfts_batch* b, const double* const* eta_new, const double* XbN, const double* dt) {
+    const int n = (int)b->ctx.size();
+    for (int id = 0; id < n; id++) {
+        if (!eta_new[id]) return bfail(b, LFTS_EINVAL, "langevin: null noise pointer");
+        Coef& co = b->h_coef[id];
+        co.XbN = XbN[id];
+        co.dt = dt[id];
+        co.eta_new = eta_new[id];
+        co.keep_new = 1;
+    }
+    if (int rc = set_active(b, all_ids(b))) return rc;
+    if (int rc = upload_coef(b)) return rc;
+    BHIPCHK(b, lfts::launch_langevin(b->d_ract, n, b->nb_max, b->d_coef, b->st));
+    return sync_batch(b);
+}
+
+int batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
+    lfts_batch* b = new lfts_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    auto cleanup = [&](int rc) {
+        lfts_batch_destroy(b);
+        return rc;
+    };
+#define BCREATE(expr)                                        \
+    do {                                                     \
+        if ((expr) != hipSuccess) return cleanup(LFTS_EHIP); \
+    } while (0)
+    BCREATE(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+    b->reps.resize(n);
+    for (int i = 0; i < n; i++) {
+        b->reps[i] = make_rep(ctxs[i]);
+        b->reps[i].id = i;
+    }
+    BCREATE(hipMalloc(&b->d_ract, sizeof(Rep) * n));
+    BCREATE(hipHostMalloc(&b->h_ract, sizeof(Rep) * n, hipHostMallocDefault));
+    BCREATE(hipMalloc(&b->d_coef, sizeof(Coef) * n));
+    BCREATE(hipHostMalloc(&b->h_coef, sizeof(Coef) * n, hipHostMallocDefault));
+    std::memset(b->h_coef, 0, sizeof(Coef) * n);
+    BCREATE(hipMalloc(&b->d_part, sizeof(double) * lfts::PSTRIDE * lfts::MAX_NB * n));
+    BCREATE(hipMalloc(&b->d_scal, sizeof(double) * lfts::SSTRIDE * n));
+    BCREATE(hipHostMalloc(&b->h_scal, sizeof(double) * lfts::SSTRIDE * n, hipHostMallocDefault));
+#undef BCREATE
+    *out = b;
+    return LFTS_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
 int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     if (!desc || !out) return LFTS_EINVAL;
     *out = nullptr;
+    if (desc->N < 2 || desc->NA < 0 || desc->NA > desc->N) return LFTS_EINVAL;
+    if (desc->nhMax < 1 || desc->nhMax > lfts::MAX_HIST) return LFTS_EINVAL;
+    for (int d = 0; d < 3; d++)
+        if (desc->m[d] < 1 || !(desc->L[d] > 0.0)) return LFTS_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LFTS_EHIP;
     lfts_ctx* c = new lfts_ctx();
     c->d = *desc;
     c->N = desc->N;
     c->NA = desc->NA;
     c->NB = desc->N - desc->NA;
     c->nhMax = desc->nhMax;
-    auto bad = [&](const char*) {
-        delete c;
-        return LFTS_EINVAL;
-    };
-    if (c->N < 2 || c->NA < 0 || c->NA > c->N) return bad("need N >= 2 and 0 <= NA <= N");
-    if (c->nhMax < 1 || c->nhMax > lfts::MAX_HIST) return bad("nhMax must be in [1, 15]");
-    for (int d = 0; d < 3; d++)
-        if (desc->m[d] < 1 || !(desc->L[d] > 0.0)) return bad("mesh sizes and box edges must be positive");
     c->M = (long)desc->m[0] * desc->m[1] * desc->m[2];
     c->Mk = (long)desc->m[0] * desc->m[1] * (desc->m[2] / 2 + 1);
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        delete c;
-        return LFTS_EHIP;
-    }
     const long M = c->M;
     int rc = LFTS_OK;
-#define CREATE_CHK(expr)                 \
-    do {                                 \
-        rc = (expr);                     \
-        if (rc) { lfts_destroy(c); return rc; } \
+#define CREATE_CHK(expr)     \
+    do {                     \
+        rc = (expr);         \
+        if (rc) {            \
+            lfts_destroy(c); \
+            return rc;       \
+        }                    \
     } while (0)
 #define CREATE_HIP(expr) CREATE_CHK((expr) == hipSuccess ? LFTS_OK : LFTS_EHIP)
-    CREATE_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     for (int d = 0; d < 3; d++) CREATE_CHK(build_axis(c, c->ax[d], desc->m[d], desc->L[d], c->N));
-    CREATE_HIP(hipMalloc(&c->w, sizeof(double) * 4 * M));
-    CREATE_HIP(hipMalloc(&c->h, sizeof(double) * 2 * M));
-    CREATE_HIP(hipMalloc(&c->q, sizeof(double) * 2 * M * (size_t)(c->N - 1)));
-    CREATE_HIP(hipMalloc(&c->tmp, sizeof(double) * 2 * M));
     {
         // LFTS_PATH=generic selects the three-pass line kernels even where the fused
         // slab pass applies (used to cross-check the two paths)
         const char* path = std::getenv("LFTS_PATH");
         const bool generic = path && std::strcmp(path, "generic") == 0;
         c->fused = !generic && lfts::slab_supported(desc->m[0], desc->m[1], desc->m[2]);
-        if (c->fused) CREATE_HIP(hipMalloc(&c->tmp2, sizeof(double) * 2 * M));
     }
+    CREATE_HIP(hipMalloc(&c->w, sizeof(double) * 4 * M));
+    CREATE_HIP(hipMalloc(&c->h, sizeof(double) * 2 * M));
+    CREATE_HIP(hipMalloc(&c->q, sizeof(double) * 2 * M * (size_t)(c->N - 1)));
+    CREATE_HIP(hipMalloc(&c->T0, sizeof(double) * 2 * M));
+    if (c->fused) CREATE_HIP(hipMalloc(&c->T1, sizeof(double) * 2 * M));
     CREATE_HIP(hipMalloc(&c->Dh, sizeof(double) * M * (size_t)(c->nhMax + 1)));
     CREATE_HIP(hipMalloc(&c->wh, sizeof(double) * M * (size_t)(c->nhMax + 1)));
     CREATE_HIP(hipMalloc(&c->noise, sizeof(double) * 2 * M));
     CREATE_HIP(hipMalloc(&c->S, sizeof(double) * c->Mk));
-    c->nblk = (int)std::min<long>((M + 255) / 256, 2048);
-    c->nred = (int)std::min<long>((M + 255) / 256, 1024);
-    CREATE_HIP(hipMalloc(&c->part, sizeof(double) * lfts::PSTRIDE * c->nred));
-    CREATE_HIP(hipMalloc(&c->scal, sizeof(double) * 64));
-    CREATE_HIP(hipHostMalloc(&c->pinned, sizeof(double) * 64, hipHostMallocDefault));
     CREATE_HIP(hipMemset(c->w, 0, sizeof(double) * 4 * M));
     CREATE_HIP(hipMemset(c->noise, 0, sizeof(double) * 2 * M));
     CREATE_HIP(hipMemset(c->S, 0, sizeof(double) * c->Mk));
@@ -400,6 +601,7 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     c->U.assign((size_t)c->nhMax * c->nhMax, 0.0);
     c->V.assign(c->nhMax, 0.0);
     c->C.assign(c->nhMax, 0.0);
+    CREATE_CHK(batch_create(&c, 1, &c->self));
 #undef CREATE_CHK
 #undef CREATE_HIP
     *out = c;
@@ -408,21 +610,16 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
 
 int lfts_destroy(lfts_ctx* c) {
     if (!c) return LFTS_EINVAL;
-    if (c->st) hipStreamSynchronize(c->st);
-    for (auto& q : c->ev_pending)
-        for (hipEvent_t e : {q.pe0, q.ch0, q.ch1, q.pe1}) c->ev_pool.push_back(e);
-    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->self) lfts_batch_destroy(c->self);
     for (auto& p : c->ax) {
-        hipFree(p.tw);
-        hipFree(p.gperm);
-        hipFree(p.perm);
-        hipFree(p.gnat);
+        (void)hipFree(p.tw);
+        (void)hipFree(p.gperm);
+        (void)hipFree(p.perm);
+        (void)hipFree(p.gnat);
     }
-    hipFree(c->tmp2);
-    hipFree(c->w); hipFree(c->h); hipFree(c->q); hipFree(c->tmp); hipFree(c->Dh); hipFree(c->wh);
-    hipFree(c->noise); hipFree(c->S); hipFree(c->zk); hipFree(c->part); hipFree(c->scal); hipFree(c->test);
-    if (c->pinned) hipHostFree(c->pinned);
-    if (c->st) hipStreamDestroy(c->st);
+    for (void* p : {(void*)c->w, (void*)c->h, (void*)c->q, (void*)c->T0, (void*)c->T1, (void*)c->Dh, (void*)c->wh,
+                    (void*)c->noise, (void*)c->S, (void*)c->zk, (void*)c->test})
+        (void)hipFree(p);
     delete c;
     return LFTS_OK;
 }
@@ -431,21 +628,21 @@ const char* lfts_last_error(const lfts_ctx* c) { return c ? c->err.c_str() : "nu
 
 int lfts_get_stream(lfts_ctx* c, void** stream) {
     if (!c || !stream) return LFTS_EINVAL;
-    *stream = (void*)c->st;
+    *stream = (void*)c->self->st;
     return LFTS_OK;
 }
 
 int lfts_set_fields(lfts_ctx* c, const double* w, int nfields) {
     if (!c || !w || (nfields != 2 && nfields != 4)) return fail(c, LFTS_EINVAL, "set_fields: nfields must be 2 or 4");
-    HIPCHK(c, hipMemcpyAsync(c->w, w, sizeof(double) * nfields * c->M, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpyAsync(c->w, w, sizeof(double) * nfields * c->M, hipMemcpyHostToDevice, c->self->st));
+    HIPCHK(c, hipStreamSynchronize(c->self->st));
     return LFTS_OK;
 }
 
 int lfts_get_fields(lfts_ctx* c, double* w, int nfields) {
     if (!c || !w || (nfields != 2 && nfields != 4)) return fail(c, LFTS_EINVAL, "get_fields: nfields must be 2 or 4");
-    HIPCHK(c, hipMemcpyAsync(w, c->w, sizeof(double) * nfields * c->M, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpyAsync(w, c->w, sizeof(double) * nfields * c->M, hipMemcpyDeviceToHost, c->self->st));
+    HIPCHK(c, hipStreamSynchronize(c->self->st));
     return LFTS_OK;
 }
 
@@ -456,133 +653,79 @@ int lfts_fields_dev(lfts_ctx* c, double** w_dev) {
 }
 
 int lfts_calc_concs(lfts_ctx* c, double* lnQ) {
-    if (check_ctx(c)) return LFTS_EINVAL;
-    int rc = calc_concs_async(c);
-    if (rc) return rc;
-    double Q;
-    if ((rc = read_scalars(c, 1, &Q))) return rc;
-    if (lnQ) *lnQ = std::log(Q);
-    if (!std::isfinite(Q) || Q <= 0.0) return fail(c, LFTS_ENUM, "partition function Q is not finite and positive");
-    return LFTS_OK;
+    if (!c) return LFTS_EINVAL;
+    return batch_calc_concs(c->self, lnQ);
 }
 
-// src/anderson.cc:24-93
 int lfts_anderson_mix(lfts_ctx* c, int maxIter, double errTol, int* k_ret, double* err_ret) {
-    if (check_ctx(c)) return LFTS_EINVAL;
-    const long M = c->M;
-    const int H = c->nhMax + 1;
-    auto DD = [&](int i, int j) -> double& { return c->DD[(size_t)i * H + j]; };
-    double err = 1.0, lambda;
-    int k, nh, rc;
-    double dd[lfts::MAX_HIST + 1];
-    for (k = 1; k < maxIter && err > errTol; k++) {
-        if ((rc = calc_concs_async(c))) return rc;
-        nh = (k < c->nhMax + 1) ? k - 1 : c->nhMax;
-        HIPCHK(c, lfts::launch_am_dots(c->w, c->Dh, M, nh, c->part, c->nred, c->st));
-        HIPCHK(c, lfts::launch_reduce_final(c->part, c->nred, nh + 1, 1.0, c->scal, c->st));
-        if ((rc = read_scalars(c, nh + 1, dd))) return rc;
-        const double S1 = dd[0];
-        err = std::pow(S1 / M, 0.5);
-        lambda = 1.0 - std::pow(0.9, double(k));
-        for (int m = 0; m <= nh; m++) DD(0, m) = dd[m];
-        AmCoef co;
-        std::memset(&co, 0, sizeof co);
-        co.lambda = lambda;
-        co.nh = 0;
-        if (k >= 2) {
-            for (int m = 1; m <= nh; m++) {
-                c->V[m - 1] = DD(0, 0) - DD(0, m);
-                for (int n = 1; n <= m; n++)
-                    c->U[(m - 1) * nh + n - 1] = c->U[(n - 1) * nh + m - 1] = DD(0, 0) - DD(0, m) - DD(0, n) + DD(n, m);
-            }
-            lu_solve(nh, c->U.data(), c->V.data(), c->C.data());
-            co.nh = nh;
-            for (int n = 0; n < nh; n++) co.C[n] = c->C[n];
-        }
-        const int n = 1 + (k - 1) % c->nhMax;
-        co.slot = n;
-        HIPCHK(c, lfts::launch_am_update(c->w, c->Dh, c->wh, M, co, c->nblk, c->st));
-        DD(n, n) = DD(0, 0);
-        for (int m = 1; m < n; m++) DD(m, n) = DD(0, m);
-        for (int m = n + 1; m <= nh; m++) DD(n, m) = DD(0, m);
-        if (std::isnan(err)) break;
-    }
-    if (k_ret) *k_ret = k;
-    if (err_ret) *err_ret = err;
-    if (!std::isfinite(err)) return fail(c, LFTS_ENUM, "Anderson residual is not finite");
-    return LFTS_OK;
+    if (!c) return LFTS_EINVAL;
+    return batch_anderson_mix(c->self, maxIter, errTol, k_ret, err_ret);
 }
 
 int lfts_langevin_init(lfts_ctx* c, const double* eta_prev) {
     if (!c || !eta_prev) return fail(c, LFTS_EINVAL, "langevin_init: null argument");
-    c->eta_prev = 0;
-    HIPCHK(c, hipMemcpyAsync(c->noise, eta_prev, sizeof(double) * c->M, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpyAsync(c->noise, eta_prev, sizeof(double) * c->M, hipMemcpyHostToDevice, c->self->st));
+    HIPCHK(c, hipStreamSynchronize(c->self->st));
     return LFTS_OK;
 }
 
 int lfts_langevin_step(lfts_ctx* c, const double* eta_new, double XbN, double dt) {
     if (!c || !eta_new) return fail(c, LFTS_EINVAL, "langevin_step: null argument");
-    const long M = c->M;
-    double* ep = c->noise + (long)c->eta_prev * M;
-    double* en = c->noise + (long)(1 - c->eta_prev) * M;
-    HIPCHK(c, hipMemcpyAsync(en, eta_new, sizeof(double) * M, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, lfts::launch_langevin(c->w, ep, en, M, XbN, dt, 0, c->nblk, c->st));
-    c->eta_prev = 1 - c->eta_prev;  // src/langevin.cc:34-36
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    return LFTS_OK;
+    double* staging = c->noise + c->M;
+    HIPCHK(c, hipMemcpyAsync(staging, eta_new, sizeof(double) * c->M, hipMemcpyHostToDevice, c->self->st));
+    const double* en[1] = {staging};
+    return batch_langevin(c->self, en, &XbN, &dt);
 }
 
 int lfts_langevin_step_dev(lfts_ctx* c, const double* eta_new_dev, double XbN, double dt) {
     if (!c || !eta_new_dev) return fail(c, LFTS_EINVAL, "langevin_step_dev: null argument");
-    double* ep = c->noise + (long)c->eta_prev * c->M;
-    HIPCHK(c, lfts::launch_langevin(c->w, ep, eta_new_dev, c->M, XbN, dt, 1, c->nblk, c->st));
-    return LFTS_OK;
+    const double* en[1] = {eta_new_dev};
+    return batch_langevin(c->self, en, &XbN, &dt);
 }
 
 int lfts_langevin_full_step_dev(lfts_ctx* c, const double* eta_new_dev, double XbN, double dt, int maxIter,
                                 double errTol, double* lnQ, int* k_ret) {
-    int rc = lfts_langevin_step_dev(c, eta_new_dev, XbN, dt);
-    if (rc) return rc;
-    rc = lfts_anderson_mix(c, maxIter, errTol, k_ret, nullptr);
-    if (rc && rc != LFTS_ENUM) return rc;
-    int rc2 = lfts_calc_concs(c, lnQ);
-    return rc ? rc : rc2;
+    if (!c) return LFTS_EINVAL;
+    return lfts_batch_full_step_dev(c->self, &eta_new_dev, &XbN, &dt, maxIter, errTol, lnQ, k_ret);
 }
 
-// src/strFunc.cc:36-46
+// src/strFunc.cc:36-46 (forward transform of W- in three line passes, |X|^2 accumulated
+// into the r2c half spectrum by the last pass)
 int lfts_sk_sample(lfts_ctx* c) {
-    if (check_ctx(c)) return LFTS_EINVAL;
+    if (!c) return LFTS_EINVAL;
+    hipStream_t st = c->self->st;
     if (!c->zk) HIPCHK(c, hipMalloc(&c->zk, sizeof(double2) * c->M));
     LineArgs z = line_args(c, 2, lfts::LINE_FFT);
     z.inR = c->w;
     z.outC = c->zk;
-    HIPCHK(c, lfts::launch_line(z, c->st));
+    HIPCHK(c, lfts::launch_line(z, st));
     LineArgs y = line_args(c, 1, lfts::LINE_FFT);
     y.inC = c->zk;
     y.outC = c->zk;
-    HIPCHK(c, lfts::launch_line(y, c->st));
+    HIPCHK(c, lfts::launch_line(y, st));
     LineArgs x = line_args(c, 0, lfts::LINE_FFT);
     x.inC = c->zk;
     x.Sacc = c->S;
-    HIPCHK(c, lfts::launch_line(x, c->st));
+    HIPCHK(c, lfts::launch_line(x, st));
+    HIPCHK(c, hipStreamSynchronize(st));
     c->nsamples++;
     return LFTS_OK;
 }
 
 int lfts_sk_get(lfts_ctx* c, double* S, int* nsamples) {
-    if (check_ctx(c)) return LFTS_EINVAL;
+    if (!c) return LFTS_EINVAL;
     if (S) {
-        HIPCHK(c, hipMemcpyAsync(S, c->S, sizeof(double) * c->Mk, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipStreamSynchronize(c->st));
+        HIPCHK(c, hipMemcpyAsync(S, c->S, sizeof(double) * c->Mk, hipMemcpyDeviceToHost, c->self->st));
+        HIPCHK(c, hipStreamSynchronize(c->self->st));
     }
     if (nsamples) *nsamples = c->nsamples;
     return LFTS_OK;
 }
 
 int lfts_sk_reset(lfts_ctx* c) {
-    if (check_ctx(c)) return LFTS_EINVAL;
-    HIPCHK(c, hipMemsetAsync(c->S, 0, sizeof(double) * c->Mk, c->st));
+    if (!c) return LFTS_EINVAL;
+    HIPCHK(c, hipMemsetAsync(c->S, 0, sizeof(double) * c->Mk, c->self->st));
+    HIPCHK(c, hipStreamSynchronize(c->self->st));
     c->nsamples = 0;
     return LFTS_OK;
 }
@@ -641,63 +784,138 @@ int lfts_sk_save(const char* fileName, const int* m, const double* L, const doub
 
 // src/lfts_simulation.cc:122-135
 int lfts_hamiltonian(lfts_ctx* c, double XbN, double* H) {
-    if (check_ctx(c)) return LFTS_EINVAL;
-    int rc = calc_concs_async(c);
-    if (rc) return rc;
-    HIPCHK(c, lfts::launch_h_partial(c->w, c->M, c->part, c->nred, c->st));
-    HIPCHK(c, lfts::launch_reduce_final(c->part, c->nred, 2, 1.0, c->scal + 1, c->st));
-    double s[3];
-    if ((rc = read_scalars(c, 3, s))) return rc;
-    const double lnQ = std::log(s[0]);
-    if (H) *H = -lnQ + (s[2] / XbN - s[1]) / c->M;
+    if (!c) return LFTS_EINVAL;
+    lfts_batch* b = c->self;
+    if (int rc = set_active(b, all_ids(b))) return rc;
+    if (int rc = calc_concs_async(b)) return rc;
+    HIPCHK(c, lfts::launch_h_sums(b->d_ract, 1, b->nb_max, b->d_part, b->st));
+    HIPCHK(c, lfts::launch_reduce(b->d_ract, 1, b->d_part, 2, 0, b->d_scal, 1, b->st));
+    if (int rc = read_scal(b)) return rc;
+    const double lnQ = std::log(scal(b, 0, 0));
+    if (H) *H = -lnQ + (scal(b, 0, 2) / XbN - scal(b, 0, 1)) / c->M;
     return std::isfinite(lnQ) ? LFTS_OK : fail(c, LFTS_ENUM, "partition function Q is not finite and positive");
 }
 
 int lfts_get_pair(lfts_ctx* c, int i, double* out) {
     if (!c || !out || i < 1 || i > c->N) return fail(c, LFTS_EINVAL, "get_pair: index out of range");
-    HIPCHK(c, hipMemcpyAsync(out, pair_ptr(c, i), sizeof(double) * 2 * c->M, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpyAsync(out, pair_ptr(c, i), sizeof(double) * 2 * c->M, hipMemcpyDeviceToHost, c->self->st));
+    HIPCHK(c, hipStreamSynchronize(c->self->st));
     return LFTS_OK;
 }
 
+// One contour step on host arrays through the line path (the slab path only runs
+// whole chains).
 int lfts_step_fwd(lfts_ctx* c, const double* q_in, double* q_out, const double* h, int i) {
     if (!c || !q_in || !q_out || !h) return fail(c, LFTS_EINVAL, "step_fwd: null argument");
     const long M = c->M;
-    if (!c->test) HIPCHK(c, hipMalloc(&c->test, sizeof(double) * 6 * M));
-    HIPCHK(c, hipMemcpyAsync(c->test, q_in, sizeof(double) * 2 * M, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->test + 2 * M, h, sizeof(double) * 2 * M, hipMemcpyHostToDevice, c->st));
-    int rc = contour_step(c, c->test, c->test + 4 * M, c->test + 2 * M, i);
-    if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(q_out, c->test + 4 * M, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    hipStream_t st = c->self->st;
+    if (!c->test) HIPCHK(c, hipMalloc(&c->test, sizeof(double) * 8 * M));
+    HIPCHK(c, hipMemcpyAsync(c->test, q_in, sizeof(double) * 2 * M, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->test + 2 * M, h, sizeof(double) * 2 * M, hipMemcpyHostToDevice, st));
+    if (int rc = line_contour_step(c, c->test, c->test + 4 * M, c->test + 2 * M, i, c->test + 6 * M, st)) return rc;
+    HIPCHK(c, hipMemcpyAsync(q_out, c->test + 4 * M, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
     return LFTS_OK;
 }
 
 int lfts_profile(lfts_ctx* c, int enable) {
-    if (check_ctx(c)) return LFTS_EINVAL;
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    if (int rc = resolve_events(c)) return rc;
-    c->prof = enable != 0;
-    c->prof_ms_chain = c->prof_ms_pe = 0.0;
-    c->prof_n_cs = c->prof_n_pe = 0;
-    return LFTS_OK;
+    if (!c) return LFTS_EINVAL;
+    return lfts_batch_profile(c->self, enable);
 }
 
 int lfts_profile_read(lfts_ctx* c, double* ms_chain, long long* n_cs, double* ms_pe, long long* n_pe) {
-    if (check_ctx(c)) return LFTS_EINVAL;
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    if (int rc = resolve_events(c)) return rc;
-    if (ms_chain) *ms_chain = c->prof_ms_chain;
-    if (n_cs) *n_cs = c->prof_n_cs;
-    if (ms_pe) *ms_pe = c->prof_ms_pe;
-    if (n_pe) *n_pe = c->prof_n_pe;
-    return LFTS_OK;
+    if (!c) return LFTS_EINVAL;
+    return lfts_batch_profile_read(c->self, ms_chain, n_cs, ms_pe, n_pe);
 }
 
 int lfts_counters(lfts_ctx* c, long long* n_pe, long long* n_cs) {
-    if (check_ctx(c)) return LFTS_EINVAL;
+    if (!c) return LFTS_EINVAL;
     if (n_pe) *n_pe = c->n_pe;
     if (n_cs) *n_cs = c->n_cs;
+    return LFTS_OK;
+}
+
+// ---- replica sets ----------------------------------------------------------------------
+
+int lfts_batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
+    if (!ctxs || n < 1 || !out) return LFTS_EINVAL;
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i]) return LFTS_EINVAL;
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return LFTS_EINVAL;
+        // work queued on the contexts' own streams finishes first
+        if (hipStreamSynchronize(ctxs[i]->self->st) != hipSuccess) return LFTS_EHIP;
+    }
+    return batch_create(ctxs, n, out);
+}
+
+int lfts_batch_destroy(lfts_batch* b) {
+    if (!b) return LFTS_EINVAL;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    for (auto& q : b->ev_pending)
+        for (hipEvent_t e : {q.pe0, q.ch0, q.ch1, q.pe1}) b->ev_pool.push_back(e);
+    for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);
+    for (void* p : {(void*)b->d_ract, (void*)b->d_coef, (void*)b->d_part, (void*)b->d_scal}) (void)hipFree(p);
+    for (void* p : {(void*)b->h_ract, (void*)b->h_coef, (void*)b->h_scal})
+        if (p) (void)hipHostFree(p);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+    return LFTS_OK;
+}
+
+const char* lfts_batch_last_error(const lfts_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int lfts_batch_get_stream(lfts_batch* b, void** stream) {
+    if (!b || !stream) return LFTS_EINVAL;
+    *stream = (void*)b->st;
+    return LFTS_OK;
+}
+
+int lfts_batch_calc_concs(lfts_batch* b, double* lnQ) {
+    if (!b) return LFTS_EINVAL;
+    return batch_calc_concs(b, lnQ);
+}
+
+int lfts_batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, double* err_ret) {
+    if (!b) return LFTS_EINVAL;
+    return batch_anderson_mix(b, maxIter, errTol, k_ret, err_ret);
+}
+
+int lfts_batch_langevin_step_dev(lfts_batch* b, const double* const* eta_new_dev, const double* XbN,
+                                 const double* dt) {
+    if (!b || !eta_new_dev || !XbN || !dt) return bfail(b, LFTS_EINVAL, "langevin_step_dev: null argument");
+    return batch_langevin(b, eta_new_dev, XbN, dt);
+}
+
+int lfts_batch_full_step_dev(lfts_batch* b, const double* const* eta_new_dev, const double* XbN, const double* dt,
+                             int maxIter, double errTol, double* lnQ, int* k_ret) {
+    if (!b || !eta_new_dev || !XbN || !dt) return bfail(b, LFTS_EINVAL, "full_step_dev: null argument");
+    int rc = batch_langevin(b, eta_new_dev, XbN, dt);
+    if (rc) return rc;
+    rc = batch_anderson_mix(b, maxIter, errTol, k_ret, nullptr);
+    if (rc && rc != LFTS_ENUM) return rc;
+    const int rc2 = batch_calc_concs(b, lnQ);
+    return rc ? rc : rc2;
+}
+
+int lfts_batch_profile(lfts_batch* b, int enable) {
+    if (!b) return LFTS_EINVAL;
+    if (int rc = sync_batch(b)) return rc;
+    b->prof = enable != 0;
+    b->prof_ms_chain = b->prof_ms_pe = 0.0;
+    b->prof_n_pass = b->prof_n_pe = 0;
+    return LFTS_OK;
+}
+
+int lfts_batch_profile_read(lfts_batch* b, double* ms_chain, long long* n_passes, double* ms_pe,
+                            long long* n_prop_evals) {
+    if (!b) return LFTS_EINVAL;
+    if (int rc = sync_batch(b)) return rc;
+    if (ms_chain) *ms_chain = b->prof_ms_chain;
+    if (n_passes) *n_passes = b->prof_n_pass;
+    if (ms_pe) *ms_pe = b->prof_ms_pe;
+    if (n_prop_evals) *n_prop_evals = b->prof_n_pe;
     return LFTS_OK;
 }
 

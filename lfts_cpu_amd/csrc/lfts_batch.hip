@@ -1,0 +1,215 @@
+// Element-wise passes and reductions of the Langevin step over a set of replicas.
+//
+// Grid (nb_max, nact): blockIdx.y picks the active replica, whose first nb workgroups
+// stride over its M mesh points (nb = blocks_for(M)). Reductions write one partial per workgroup in a fixed order and a
+// second kernel sums them, so results are identical from run to run and identical
+// between a replica run alone and the same replica run inside a larger set.
+#include <hip/hip_runtime.h>
+
+#include "lfts_internal.h"
+
+namespace lfts {
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Sum of v over a 256-thread workgroup; valid in thread 0.
+__device__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i];
+    return s;
+}
+
+__device__ __forceinline__ const double* pair_ptr(const Rep& R, int i) {
+    return (i == 1) ? R.h : R.q + 2L * (i - 2) * R.M;
+}
+
+__device__ __forceinline__ double* part_row(double* part, int slot) {
+    return part + ((long)slot * MAX_NB + blockIdx.x) * PSTRIDE;
+}
+
+#define FOR_POINTS(r, R) for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < (R).M; r += (long)(R).nb * blockDim.x)
+#define REPLICA(R)                              \
+    const Rep R = reps[blockIdx.y];             \
+    if ((int)blockIdx.x >= R.nb) return
+
+}  // namespace
+
+// src/diblock.cc:64-72 (prepare_h): hA = exp(-(W-+w+)/N), hB = exp(-(-W-+w+)/N)
+__global__ __launch_bounds__(256) void k_prepare_h(const Rep* __restrict__ reps) {
+    REPLICA(R);
+    FOR_POINTS(r, R) {
+        const double wm = R.w[r], wp = R.w[r + R.M];
+        R.h[r] = exp(-(wm + wp) / R.N);
+        R.h[r + R.M] = exp(-(-wm + wp) / R.N);
+    }
+}
+
+// partial sums of q_N (src/diblock.cc:46-48)
+__global__ __launch_bounds__(256) void k_q_sum(const Rep* __restrict__ reps, double* __restrict__ part) {
+    __shared__ double red[8];
+    REPLICA(R);
+    const double* qN = pair_ptr(R, R.N);
+    double s = 0.0;
+    FOR_POINTS(r, R) s += qN[r];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part_row(part, blockIdx.y)[0] = s;
+}
+
+// scal[rid][off + c] = sum over the replica's partials (divided by M when asked)
+__global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, const double* __restrict__ part,
+                                                int ncomp, int div_by_M, double* __restrict__ scal, int off) {
+    __shared__ double red[8];
+    const int slot = blockIdx.x;
+    const Rep R = reps[slot];
+    const double* p = part + (long)slot * MAX_NB * PSTRIDE;
+    for (int c = 0; c < ncomp; c++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < R.nb; b += blockDim.x) s += p[(long)b * PSTRIDE + c];
+        s = block_sum(s, red);
+        if (threadIdx.x == 0) scal[(long)R.id * SSTRIDE + off + c] = div_by_M ? s / R.M : s;
+    }
+}
+
+// src/diblock.cc:51-53,76-95: phiA = sum_{i<=NA} q_i q+_i, phiB = sum_{i>NA} q_i q+_i,
+// normalised by N Q h and written as phi- = phiA - phiB, phi+ = phiA + phiB.
+__global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const double* __restrict__ scal) {
+    REPLICA(R);
+    const double Q = scal[(long)R.id * SSTRIDE];
+    const long M = R.M;
+    FOR_POINTS(r, R) {
+        double pA = 0.0, pB = 0.0;
+        for (int i = 1; i <= R.NA; i++) pA += pair_ptr(R, i)[r] * pair_ptr(R, R.N + 1 - i)[M + r];
+        for (int i = R.NA + 1; i <= R.N; i++) pB += pair_ptr(R, i)[r] * pair_ptr(R, R.N + 1 - i)[M + r];
+        const double phA = pA / (R.N * Q * R.h[r]);
+        const double phB = pB / (R.N * Q * R.h[r + M]);
+        R.w[2 * M + r] = phA - phB;
+        R.w[3 * M + r] = phA + phB;
+    }
+}
+
+// src/anderson.cc:35-55: DD[0][m] = <Dh0, Dh_m>, Dh0 = phi+ - 1 (m = 0 uses Dh0 itself)
+__global__ __launch_bounds__(256) void k_am_dots(const Rep* __restrict__ reps, const Coef* __restrict__ coef,
+                                                 double* __restrict__ part) {
+    __shared__ double red[8];
+    REPLICA(R);
+    const int nh = coef[R.id].nh;
+    const long M = R.M;
+    double acc[MAX_HIST + 1];
+#pragma unroll
+    for (int m = 0; m <= MAX_HIST; m++) acc[m] = 0.0;
+    FOR_POINTS(r, R) {
+        const double d0 = R.w[3 * M + r] - 1.0;
+        acc[0] += d0 * d0;
+#pragma unroll
+        for (int m = 1; m <= MAX_HIST; m++)
+            if (m <= nh) acc[m] += d0 * R.Dh[(long)m * M + r];
+    }
+    double* out = part_row(part, blockIdx.y);
+#pragma unroll
+    for (int m = 0; m <= MAX_HIST; m++) {
+        if (m > nh) break;
+        const double s = block_sum(acc[m], red);
+        if (threadIdx.x == 0) out[m] = s;
+    }
+}
+
+// src/anderson.cc:57-89: w+ += lambda Dh0 + sum_n C_n((wh_n + lambda Dh_n) - (wh0 + lambda Dh0)),
+// then (Dh0, wh0) become history slot `slot`.
+__global__ __launch_bounds__(256) void k_am_update(const Rep* __restrict__ reps, const Coef* __restrict__ coef) {
+    REPLICA(R);
+    const Coef& c = coef[R.id];
+    const double lambda = c.lambda;
+    const int nh = c.nh, slot = c.slot;
+    const long M = R.M;
+    FOR_POINTS(r, R) {
+        const double d0 = R.w[3 * M + r] - 1.0;
+        const double w0 = R.w[M + r];
+        const double base = w0 + lambda * d0;
+        double wn = base;
+        for (int n = 1; n <= nh; n++) wn += c.C[n - 1] * ((R.wh[(long)n * M + r] + lambda * R.Dh[(long)n * M + r]) - base);
+        R.w[M + r] = wn;
+        R.Dh[(long)slot * M + r] = d0;
+        R.wh[(long)slot * M + r] = w0;
+    }
+}
+
+// src/langevin.cc:31: W- += -(phi- + 2 W-/XbN) dt + 0.5 (eta_prev + eta_new); with keep_new the
+// new noise replaces eta_prev in the same pass (the pointer swap of src/langevin.cc:34-36).
+__global__ __launch_bounds__(256) void k_langevin(const Rep* __restrict__ reps, const Coef* __restrict__ coef) {
+    REPLICA(R);
+    const Coef& c = coef[R.id];
+    const double XbN = c.XbN, dt = c.dt;
+    const double* en = c.eta_new;
+    const int keep = c.keep_new;
+    const long M = R.M;
+    FOR_POINTS(r, R) {
+        const double wm = R.w[r];
+        const double a = R.eta_prev[r], b = en[r];
+        R.w[r] = wm + (-(R.w[2 * M + r] + 2 * wm / XbN) * dt + 0.5 * (a + b));
+        if (keep) R.eta_prev[r] = b;
+    }
+}
+
+// src/lfts_simulation.cc:130-131: partials of sum w+ and sum W-^2
+__global__ __launch_bounds__(256) void k_h_sums(const Rep* __restrict__ reps, double* __restrict__ part) {
+    __shared__ double red[8];
+    REPLICA(R);
+    const long M = R.M;
+    double sp = 0.0, s2 = 0.0;
+    FOR_POINTS(r, R) {
+        sp += R.w[M + r];
+        s2 += R.w[r] * R.w[r];
+    }
+    const double a = block_sum(sp, red);
+    const double b = block_sum(s2, red);
+    if (threadIdx.x == 0) {
+        part_row(part, blockIdx.y)[0] = a;
+        part_row(part, blockIdx.y)[1] = b;
+    }
+}
+
+#define LAUNCH(kernel, grid, ...)                                            \
+    do {                                                                     \
+        hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, __VA_ARGS__);     \
+        return hipGetLastError();                                            \
+    } while (0)
+
+hipError_t launch_prepare_h(const Rep* reps, int nact, int nb_max, hipStream_t st) {
+    LAUNCH(k_prepare_h, dim3(nb_max, nact), reps);
+}
+hipError_t launch_q_sum(const Rep* reps, int nact, int nb_max, double* part, hipStream_t st) {
+    LAUNCH(k_q_sum, dim3(nb_max, nact), reps, part);
+}
+hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncomp, int div_by_M, double* scal,
+                         int scal_off, hipStream_t st) {
+    LAUNCH(k_reduce, dim3(nact), reps, part, ncomp, div_by_M, scal, scal_off);
+}
+hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, hipStream_t st) {
+    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal);
+}
+hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st) {
+    LAUNCH(k_am_dots, dim3(nb_max, nact), reps, coef, part);
+}
+hipError_t launch_am_update(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st) {
+    LAUNCH(k_am_update, dim3(nb_max, nact), reps, coef);
+}
+hipError_t launch_langevin(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st) {
+    LAUNCH(k_langevin, dim3(nb_max, nact), reps, coef);
+}
+hipError_t launch_h_sums(const Rep* reps, int nact, int nb_max, double* part, hipStream_t st) {
+    LAUNCH(k_h_sums, dim3(nb_max, nact), reps, part);
+}
+
+}  // namespace lfts

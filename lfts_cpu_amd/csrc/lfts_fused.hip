@@ -1,4 +1,5 @@
-// Fused contour step for cubic meshes of 32^3 and 64^3 points: one launch per step.
+// Fused contour step for cubic meshes of 32^3, 48^3 and 64^3 points: one launch per
+// step over every active replica of that mesh size.
 //
 // The contour step q_{i+1} = h ⊙ C q_i (src/step.cc:35-65) uses the 3D convolution
 // C = Cx Cy Cz, separable into 1D circular convolutions along x, y and z. A slab of the
@@ -81,9 +82,55 @@ __device__ __forceinline__ void dft8(double2* v) {
     v[1] = b0; v[3] = b1; v[5] = b2; v[7] = b3;
 }
 
+// natural-order radix-3 DFT
+template <bool INV>
+__device__ __forceinline__ void dft3(double2& a, double2& b, double2& c) {
+    const double s3 = 0.86602540378443864676;  // sqrt(3)/2
+    const double2 t = c_add(b, c), d = c_sub(b, c);
+    const double2 m = make_double2(a.x - 0.5 * t.x, a.y - 0.5 * t.y);
+    const double2 isd = make_double2(-s3 * d.y, s3 * d.x);  // i (sqrt3/2) d
+    a = c_add(a, t);
+    b = INV ? c_add(m, isd) : c_sub(m, isd);
+    c = INV ? c_sub(m, isd) : c_add(m, isd);
+}
+
+// natural-order radix-12 DFT as 4 x 3: x[n2 + 3 n1] -> radix-4 over n1, twiddle
+// w12^{n2 k1}, radix-3 over n2 -> X[k1 + 4 k2]
+template <bool INV>
+__device__ __forceinline__ void dft12(double2* v) {
+    const double s3 = 0.86602540378443864676;
+    double2 y[3][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; n2++) {
+        y[n2][0] = v[n2];
+        y[n2][1] = v[n2 + 3];
+        y[n2][2] = v[n2 + 6];
+        y[n2][3] = v[n2 + 9];
+        dft4<INV>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+    }
+    // w12^e (forward) for e = 1, 2, 3, 4, 6; the inverse uses the conjugates
+    const double2 w1 = make_double2(s3, INV ? 0.5 : -0.5);
+    const double2 w2 = make_double2(0.5, INV ? s3 : -s3);
+    const double2 w4 = make_double2(-0.5, INV ? s3 : -s3);
+    y[1][1] = c_mul(y[1][1], w1);
+    y[1][2] = c_mul(y[1][2], w2);
+    y[1][3] = c_rot<INV>(y[1][3]);             // w12^3 = -i
+    y[2][1] = c_mul(y[2][1], w2);
+    y[2][2] = c_mul(y[2][2], w4);
+    y[2][3] = make_double2(-y[2][3].x, -y[2][3].y);  // w12^6 = -1
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        dft3<INV>(y[0][k1], y[1][k1], y[2][k1]);
+        v[k1] = y[0][k1];
+        v[k1 + 4] = y[1][k1];
+        v[k1 + 8] = y[2][k1];
+    }
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void dft(double2* v) {
-    if constexpr (R == 8) dft8<INV>(v);
+    if constexpr (R == 12) dft12<INV>(v);
+    else if constexpr (R == 8) dft8<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
     else if constexpr (R == 2) dft2<INV>(v[0], v[1]);
 }
@@ -132,64 +179,79 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 
 }  // namespace
 
-template <int N, int R1, int R2>
-__global__ __launch_bounds__((N / 2) * R2) void k_slab(SlabArgs a) {
-    static_assert(R1 * R2 == N && R1 % R2 == 0, "unsupported factorisation");
-    constexpr int H = N / 2;           // complex lines per plane
+template <int E, int R1, int R2>
+__global__ __launch_bounds__((E / 2) * R2) void k_slab(const Rep* __restrict__ reps, const double2* __restrict__ tw,
+                                                     int i) {
+    static_assert(R1 * R2 == E && R1 % R2 == 0, "unsupported factorisation");
+    constexpr int H = E / 2;           // complex lines per plane
     constexpr int U = R1 / R2;
-    constexpr int XP = N + 1;          // exchange row pitch (complex)
-    constexpr int RP = N + 1;          // real plane row pitch (doubles)
+    constexpr int XP = E + 1;          // exchange row pitch (complex)
+    constexpr int RP = E + 1;          // real plane row pitch (doubles)
     constexpr int XBYTES = H * XP * 16;
-    constexpr int RBYTES = N * RP * 8;
+    constexpr int RBYTES = E * RP * 8;
     constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
     __shared__ __align__(16) unsigned char smem[BYTES];
     double2* xbuf = reinterpret_cast<double2*>(smem);
     double* rp = reinterpret_cast<double*>(smem);
 
+    const int slot = blockIdx.x / E, plane = blockIdx.x - slot * E;
+    const Rep R = reps[slot];
+    if (i > R.N - 1) return;  // this replica's chain is complete
+    const bool xslab = (i % 2 == 0);
+    const bool finish = (i >= 1), cont = (i < R.N - 1);
+    const long M = R.M;
+    const long plane_stride = xslab ? (long)E * E : E;
+    const long Sa = xslab ? E : (long)E * E;
+    const double* Tin = (i == 0) ? R.h : ((i & 1) ? R.T0 : R.T1);
+    double* Tout = (i & 1) ? R.T1 : R.T0;
+    const double* ga = xslab ? R.g[1] : R.g[0];
+    const double* gb = R.g[2];
+
     const int t = threadIdx.x;
     const int f = blockIdx.y;
-    const long pbase = (long)blockIdx.x * a.plane_stride;  // plane origin within a field
-    const long base = (long)f * a.M + pbase;
+    const long pbase = (long)plane * plane_stride;  // plane origin within a field
+    const long base = (long)f * M + pbase;
     const int c = t % H, j = t / H;    // layout A: a-lines, column pair (c, c + H), a = j + R2 k
     const int r = t / R2, jb = t % R2; // layout B: z-lines, row pair (r, r + H), b = jb + R2 k
 
     // everything this thread reads is issued up front: the partial pair, the Boltzmann
-    // factor, and its twiddles and kernel values (layout A and layout B)
+    // factor for its field at this bead (src/step.cc:53-64), twiddles and kernel values
     double2 z[R1], hz[R1];
-    const double* hsel = f ? a.hqd : a.hq;
+    const double* hsel = f ? ((i < R.NB) ? R.h + M : R.h) : ((i < R.NA) ? R.h : R.h + M);
 #pragma unroll
     for (int k = 0; k < R1; k++) {
-        const long p = pbase + (long)(j + R2 * k) * a.Sa + c;
-        z[k] = make_double2(a.Tin[(long)f * a.M + p], a.Tin[(long)f * a.M + p + H]);
-        if (a.finish) hz[k] = make_double2(hsel[p], hsel[p + H]);
+        const long p = pbase + (long)(j + R2 * k) * Sa + c;
+        z[k] = make_double2(Tin[(long)f * M + p], Tin[(long)f * M + p + H]);
+        if (finish) hz[k] = make_double2(hsel[p], hsel[p + H]);
     }
     double2 twA[R1], twB[R1];
     double gA[R1], gB[R1];
 #pragma unroll
     for (int k = 0; k < R1; k++) {
-        twA[k] = a.tw[j * k];
-        twB[k] = a.tw[jb * k];
+        twA[k] = tw[j * k];
+        twB[k] = tw[jb * k];
     }
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
         for (int ff = 0; ff < R2; ff++) {
-            gA[u * R2 + ff] = a.ga[j * U + u + R1 * ff];
-            gB[u * R2 + ff] = a.gb[jb * U + u + R1 * ff];
+            gA[u * R2 + ff] = ga[j * U + u + R1 * ff];
+            gB[u * R2 + ff] = gb[jb * U + u + R1 * ff];
         }
 
-    if (a.finish) {
+    if (finish) {
         conv_line<R1, R2>(z, xbuf + c * XP, j, twA, gA);
+        double* qout = R.q + 2L * (i - 1) * M;  // pair i+1
 #pragma unroll
         for (int k = 0; k < R1; k++) {
-            const long p = (long)f * a.M + pbase + (long)(j + R2 * k) * a.Sa + c;
+            const long p = base + (long)(j + R2 * k) * Sa + c;
             z[k].x *= hz[k].x;
             z[k].y *= hz[k].y;
-            a.qout[p] = z[k].x;
-            a.qout[p + H] = z[k].y;
+            qout[p] = z[k].x;
+            qout[p + H] = z[k].y;
         }
     }
-    if (!a.cont) return;
+    if (!cont) return;
 
     conv_line<R1, R2>(z, xbuf + c * XP, j, twA, gA);
     // re-pair the real plane: rows (r, r + H) become the complex z-lines
@@ -210,22 +272,24 @@ __global__ __launch_bounds__((N / 2) * R2) void k_slab(SlabArgs a) {
     conv_line<R1, R2>(w, xbuf + r * XP, jb, twB, gB);
 #pragma unroll
     for (int k = 0; k < R1; k++) {
-        const long p = base + (long)r * a.Sa + jb + R2 * k;
-        a.Tout[p] = w[k].x;
-        a.Tout[p + (long)H * a.Sa] = w[k].y;
+        const long p = base + (long)r * Sa + jb + R2 * k;
+        Tout[p] = w[k].x;
+        Tout[p + (long)H * Sa] = w[k].y;
     }
 }
 
 bool slab_supported(int n0, int n1, int n2) {
-    return n0 == n1 && n1 == n2 && (n0 == 32 || n0 == 64);
+    return n0 == n1 && n1 == n2 && (n0 == 32 || n0 == 48 || n0 == 64);
 }
 
-hipError_t launch_slab(const SlabArgs& a, int n, int nplanes, hipStream_t st) {
-    const dim3 grid(nplanes, 2);
+hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st) {
+    const dim3 grid(nact * n, 2);
     if (n == 64) {
-        hipLaunchKernelGGL((k_slab<64, 8, 8>), grid, dim3(32 * 8), 0, st, a);
+        hipLaunchKernelGGL((k_slab<64, 8, 8>), grid, dim3(32 * 8), 0, st, reps, tw, i);
+    } else if (n == 48) {
+        hipLaunchKernelGGL((k_slab<48, 12, 4>), grid, dim3(24 * 4), 0, st, reps, tw, i);
     } else if (n == 32) {
-        hipLaunchKernelGGL((k_slab<32, 8, 4>), grid, dim3(16 * 4), 0, st, a);
+        hipLaunchKernelGGL((k_slab<32, 8, 4>), grid, dim3(16 * 4), 0, st, reps, tw, i);
     } else {
         return hipErrorInvalidValue;
     }

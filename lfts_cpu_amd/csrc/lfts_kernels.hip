@@ -1,4 +1,4 @@
-// Device kernels of the L-FTS Langevin step for gfx950 (FP64 throughout).
+// Generic line kernels of the L-FTS contour step for gfx950 (FP64 throughout).
 //
 // The contour step of the reference (src/step.cc:35-65) is
 //     q_{i+1} = h ⊙ c2r( g ⊙ r2c(q_i) ),   g(k) = exp(-|K|^2/(6N)) / M.
@@ -205,200 +205,10 @@ __global__ __launch_bounds__(256) void k_line(LineArgs a) {
     }
 }
 
-// ------------------------------------------------------------------------------------
-// Reductions: per-workgroup partials in a fixed order, then one workgroup sums them,
-// so results are identical from run to run.
-// ------------------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-// Sum of v over the workgroup (256 threads); valid in thread 0.
-__device__ double block_sum(double v, double* red) {
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) red[wv] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i];
-    return s;
-}
-
-// out[c] = (sum_b part[b*PSTRIDE + c]) / div, c < ncomp
-__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ part, int nblk, int ncomp,
-                                                      double div, double* __restrict__ out) {
-    __shared__ double red[8];
-    for (int c = 0; c < ncomp; c++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(long)b * PSTRIDE + c];
-        s = block_sum(s, red);
-        if (threadIdx.x == 0) out[c] = s / div;
-    }
-}
-
-// src/diblock.cc:64-72 (prepare_h): hA = exp(-(W-+w+)/N), hB = exp(-(-W-+w+)/N)
-__global__ __launch_bounds__(256) void k_prepare_h(const double* __restrict__ w, double* __restrict__ h, long M, int N) {
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < M; r += (long)gridDim.x * blockDim.x) {
-        const double wm = w[r], wp = w[r + M];
-        h[r] = exp(-(wm + wp) / N);
-        h[r + M] = exp(-(-wm + wp) / N);
-    }
-}
-
-// partial sums of x[0..M)
-__global__ __launch_bounds__(256) void k_sum_partial(const double* __restrict__ x, long M, double* __restrict__ part) {
-    __shared__ double red[8];
-    double s = 0.0;
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < M; r += (long)gridDim.x * blockDim.x) s += x[r];
-    s = block_sum(s, red);
-    if (threadIdx.x == 0) part[(long)blockIdx.x * PSTRIDE] = s;
-}
-
-__device__ __forceinline__ const double* pair_ptr(const double* h, const double* q, int i, long M) {
-    return (i == 1) ? h : q + 2L * (i - 2) * M;
-}
-
-// src/diblock.cc:51-53,76-95: phiA = sum_{i<=NA} q_i q+_i, phiB = sum_{i>NA} q_i q+_i,
-// normalised by N Q h, written as phi- = phiA - phiB, phi+ = phiA + phiB.
-__global__ __launch_bounds__(256) void k_phi(const double* __restrict__ h, const double* __restrict__ q,
-                                             double* __restrict__ w, const double* __restrict__ Qp, long M,
-                                             int N, int NA) {
-    const double Q = Qp[0];
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < M; r += (long)gridDim.x * blockDim.x) {
-        double pA = 0.0, pB = 0.0;
-        for (int i = 1; i <= NA; i++)
-            pA += pair_ptr(h, q, i, M)[r] * pair_ptr(h, q, N + 1 - i, M)[M + r];
-        for (int i = NA + 1; i <= N; i++)
-            pB += pair_ptr(h, q, i, M)[r] * pair_ptr(h, q, N + 1 - i, M)[M + r];
-        const double phA = pA / (N * Q * h[r]);
-        const double phB = pB / (N * Q * h[r + M]);
-        w[2 * M + r] = phA - phB;
-        w[3 * M + r] = phA + phB;
-    }
-}
-
-// src/anderson.cc:35-55: DD[0][m] = <Dh0, Dh_m>, Dh0 = phi+ - 1 (m = 0 uses Dh0 itself)
-__global__ __launch_bounds__(256) void k_am_dots(const double* __restrict__ w, const double* __restrict__ Dh,
-                                                 long M, int nh, double* __restrict__ part) {
-    __shared__ double red[8];
-    double acc[MAX_HIST + 1];
-#pragma unroll
-    for (int m = 0; m <= MAX_HIST; m++) acc[m] = 0.0;
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < M; r += (long)gridDim.x * blockDim.x) {
-        const double d0 = w[3 * M + r] - 1.0;
-        acc[0] += d0 * d0;
-#pragma unroll
-        for (int m = 1; m <= MAX_HIST; m++)
-            if (m <= nh) acc[m] += d0 * Dh[(long)m * M + r];
-    }
-#pragma unroll
-    for (int m = 0; m <= MAX_HIST; m++) {
-        if (m > nh) break;
-        const double s = block_sum(acc[m], red);
-        if (threadIdx.x == 0) part[(long)blockIdx.x * PSTRIDE + m] = s;
-    }
-}
-
-// src/anderson.cc:57-89: w+ += lambda Dh0 + sum_n C_n((wh_n + lambda Dh_n) - (wh0 + lambda Dh0));
-// then (Dh0, wh0) become history slot `slot`.
-__global__ __launch_bounds__(256) void k_am_update(double* __restrict__ w, double* __restrict__ Dh,
-                                                   double* __restrict__ wh, long M, AmCoef c) {
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < M; r += (long)gridDim.x * blockDim.x) {
-        const double d0 = w[3 * M + r] - 1.0;
-        const double w0 = w[M + r];
-        double wn = w0 + c.lambda * d0;
-        const double base = w0 + c.lambda * d0;
-        for (int n = 1; n <= c.nh; n++)
-            wn += c.C[n - 1] * ((wh[(long)n * M + r] + c.lambda * Dh[(long)n * M + r]) - base);
-        w[M + r] = wn;
-        Dh[(long)c.slot * M + r] = d0;
-        wh[(long)c.slot * M + r] = w0;
-    }
-}
-
-// src/langevin.cc:31: W- += -(phi- + 2 W-/XbN) dt + 0.5 (eta_prev + eta_new)
-// When keep_new is set, eta_new is copied over eta_prev in the same pass (the swap of
-// src/langevin.cc:34-36 for noise that lives in a caller-owned buffer).
-__global__ __launch_bounds__(256) void k_langevin(double* __restrict__ w, double* __restrict__ ep,
-                                                  const double* __restrict__ en, long M, double XbN, double dt,
-                                                  int keep_new) {
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < M; r += (long)gridDim.x * blockDim.x) {
-        const double wm = w[r];
-        const double a = ep[r], b = en[r];
-        w[r] = wm + (-(w[2 * M + r] + 2 * wm / XbN) * dt + 0.5 * (a + b));
-        if (keep_new) ep[r] = b;
-    }
-}
-
-// src/lfts_simulation.cc:130-131: partials of sum w+ and sum W-^2
-__global__ __launch_bounds__(256) void k_h_partial(const double* __restrict__ w, long M, double* __restrict__ part) {
-    __shared__ double red[8];
-    double sp = 0.0, s2 = 0.0;
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < M; r += (long)gridDim.x * blockDim.x) {
-        sp += w[M + r];
-        s2 += w[r] * w[r];
-    }
-    const double a = block_sum(sp, red);
-    const double b = block_sum(s2, red);
-    if (threadIdx.x == 0) {
-        part[(long)blockIdx.x * PSTRIDE] = a;
-        part[(long)blockIdx.x * PSTRIDE + 1] = b;
-    }
-}
-
-// ------------------------------------------------------------------------------------
-// launch wrappers (host side)
-// ------------------------------------------------------------------------------------
 hipError_t launch_line(const LineArgs& a, hipStream_t st) {
     const long nblk = (a.nlines + a.T - 1) / a.T;
     const size_t lds = sizeof(double2) * (size_t)a.T * a.pitch;
     hipLaunchKernelGGL(k_line, dim3((unsigned)nblk), dim3(256), lds, st, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_reduce_final(const double* part, int nblk, int ncomp, double div, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, part, nblk, ncomp, div, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_prepare_h(const double* w, double* h, long M, int N, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_prepare_h, dim3(nblk), dim3(256), 0, st, w, h, M, N);
-    return hipGetLastError();
-}
-
-hipError_t launch_sum_partial(const double* x, long M, double* part, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_sum_partial, dim3(nblk), dim3(256), 0, st, x, M, part);
-    return hipGetLastError();
-}
-
-hipError_t launch_phi(const double* h, const double* q, double* w, const double* Qp, long M, int N, int NA,
-                      int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_phi, dim3(nblk), dim3(256), 0, st, h, q, w, Qp, M, N, NA);
-    return hipGetLastError();
-}
-
-hipError_t launch_am_dots(const double* w, const double* Dh, long M, int nh, double* part, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_am_dots, dim3(nblk), dim3(256), 0, st, w, Dh, M, nh, part);
-    return hipGetLastError();
-}
-
-hipError_t launch_am_update(double* w, double* Dh, double* wh, long M, const AmCoef& c, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_am_update, dim3(nblk), dim3(256), 0, st, w, Dh, wh, M, c);
-    return hipGetLastError();
-}
-
-hipError_t launch_langevin(double* w, double* ep, const double* en, long M, double XbN, double dt, int keep_new,
-                           int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_langevin, dim3(nblk), dim3(256), 0, st, w, ep, en, M, XbN, dt, keep_new);
-    return hipGetLastError();
-}
-
-hipError_t launch_h_partial(const double* w, long M, double* part, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_h_partial, dim3(nblk), dim3(256), 0, st, w, M, part);
     return hipGetLastError();
 }
 

@@ -1,0 +1,107 @@
+"""Replica sets: a set of contexts with mixed meshes (the slab path at 32^3, 48^3 and
+64^3, the line path elsewhere) gives every member exactly the result the member gets
+alone, and the 48^3 slab path matches the oracle."""
+import numpy as np
+import pytest
+
+import lfts_cpu_amd as L
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = [
+    # (m, L, N, NA, seed)
+    ((32, 32, 32), (3.6, 3.6, 3.6), 16, 7, 1),
+    ((48, 48, 48), (4.1, 4.1, 4.1), 16, 9, 2),
+    ((64, 64, 64), (4.4, 4.4, 4.4), 12, 6, 3),
+    ((12, 10, 8), (3.0, 2.5, 2.0), 14, 5, 4),
+    ((32, 32, 32), (3.9, 3.9, 3.9), 20, 12, 5),
+]
+
+
+def _fields(M, seed, a=2.0, b=1.0):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([rng.normal(0, a, M), rng.normal(0, b, M)])
+
+
+def _make(members):
+    ctxs, ws = [], []
+    for m, Lb, N, NA, seed in members:
+        M = int(np.prod(m))
+        c = L.Context(N, NA, m, Lb)
+        w = _fields(M, seed)
+        c.set_fields(w)
+        ctxs.append(c)
+        ws.append(w)
+    return ctxs, ws
+
+
+def test_batch_calc_concs_equals_members_alone(gpu):
+    ctxs, ws = _make(MEMBERS)
+    b = L.Batch(ctxs)
+    lnq = b.calc_concs()
+    got = [c.get_fields(4) for c in ctxs]
+    b.close()
+    alone, _ = _make(MEMBERS)
+    for i, c in enumerate(alone):
+        assert c.calc_concs() == lnq[i]
+        assert np.array_equal(c.get_fields(4), got[i]), i
+
+
+def test_batch_anderson_equals_members_alone(gpu):
+    ctxs, _ = _make(MEMBERS)
+    b = L.Batch(ctxs)
+    k, err = b.anderson_mix(200, 1e-4)
+    got = [c.get_fields(4) for c in ctxs]
+    b.close()
+    assert len(set(k.tolist())) > 1  # members converge after different iteration counts
+    alone, _ = _make(MEMBERS)
+    for i, c in enumerate(alone):
+        ki, erri = c.anderson_mix(200, 1e-4)
+        assert ki == k[i] and erri == err[i], i
+        assert np.array_equal(c.get_fields(4), got[i]), i
+
+
+def test_batch_full_steps_equal_members_alone(gpu):
+    import torch
+
+    ctxs, _ = _make(MEMBERS)
+    for c in ctxs:
+        c.langevin_init(np.random.default_rng(99).normal(0, 1.0, c.M))
+    b = L.Batch(ctxs)
+    etas = [[torch.from_numpy(np.random.default_rng(100 * s + i).normal(0, 1.0, c.M)).cuda() for i, c in
+             enumerate(ctxs)] for s in range(2)]
+    torch.cuda.synchronize()
+    XbN = [12.0, 14.0, 16.0, 13.0, 15.0]
+    out = []
+    for s in range(2):
+        out.append(b.full_step_dev([e.data_ptr() for e in etas[s]], XbN, 0.5))
+    got = [c.get_fields(4) for c in ctxs]
+    b.close()
+    alone, _ = _make(MEMBERS)
+    for i, c in enumerate(alone):
+        c.langevin_init(np.random.default_rng(99).normal(0, 1.0, c.M))
+        for s in range(2):
+            lnq, k = c.full_step_dev(etas[s][i].data_ptr(), XbN[i], 0.5)
+            assert lnq == out[s][0][i] and k == out[s][1][i], (i, s)
+        assert np.array_equal(c.get_fields(4), got[i]), i
+
+
+def test_slab_48_against_oracle(gpu):
+    m, Lb, N, NA = (48, 48, 48), (4.1, 4.1, 4.1), 10, 4
+    M = 48 ** 3
+    w = np.zeros(4 * M)
+    w[:2 * M] = _fields(M, 7, 3.0, 1.0)
+    ctx = L.Context(N, NA, m, Lb)
+    ctx.set_fields(w[:2 * M])
+    lnq = ctx.calc_concs()
+    got = ctx.get_fields(4)
+    d = O.Diblock(N, NA, m, Lb)
+    wo = w.copy()
+    lnq_ref = d.calc_concs(wo)
+    assert abs(lnq - lnq_ref) <= 1e-10 * max(1.0, abs(lnq_ref))
+    rel = np.abs(got[2 * M:] - wo[2 * M:]).max() / np.abs(wo[2 * M:]).max()
+    assert rel < 1e-10
+    for i in (2, N // 2, N):
+        p, po = ctx.pair(i), d.pair(i)
+        assert np.abs(p - po).max() / np.abs(po).max() < 1e-10, i
