@@ -190,6 +190,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batched-steps", type=int, default=3, help="timed steps of the replica-set leg (0 skips it)")
     ap.add_argument("--batched-quick", action="store_true", help="8 members instead of 64 (smoke runs)")
+    ap.add_argument("--only-batched", action="store_true", help="run only the replica-set leg (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -199,6 +200,10 @@ def main():
     import torch
 
     import lfts_cpu_amd as L
+
+    if args.only_batched:
+        print(json.dumps({"batched": batched_leg(args.batched_steps, 1, L, torch, quick=args.batched_quick)}))
+        return
 
     N, NA, XbN, C, dt = 90, 45, 17.0, 50.0, 0.5
     m = (args.mesh,) * 3

@@ -374,7 +374,7 @@ int calc_concs_async(lfts_batch* b) {
     // slab passes: i = 0 prologue, i = 1..N-1 contour steps (src/diblock.cc:43)
     for (int i = 0; i < Nmax; i++)
         for (const auto& g : b->groups) {
-            BHIPCHK(b, lfts::launch_slab(b->d_ract + g.off, g.cnt, g.edge, i, g.tw, b->st));
+            BHIPCHK(b, lfts::launch_slab(b->d_ract + g.off, g.cnt, g.edge, i, g.tw, b->st, b->h_ract + g.off));
             if (i > 0) npasses++;
         }
     for (int id : b->line_reps) {

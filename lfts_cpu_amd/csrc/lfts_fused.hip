@@ -180,8 +180,7 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 }  // namespace
 
 template <int E, int R1, int R2>
-__global__ __launch_bounds__((E / 2) * R2) void k_slab(const Rep* __restrict__ reps, const double2* __restrict__ tw,
-                                                     int i) {
+__device__ __forceinline__ void slab_pass(const Rep& R, int plane, const double2* __restrict__ tw, int i) {
     static_assert(R1 * R2 == E && R1 % R2 == 0, "unsupported factorisation");
     constexpr int H = E / 2;           // complex lines per plane
     constexpr int U = R1 / R2;
@@ -194,8 +193,6 @@ __global__ __launch_bounds__((E / 2) * R2) void k_slab(const Rep* __restrict__ r
     double2* xbuf = reinterpret_cast<double2*>(smem);
     double* rp = reinterpret_cast<double*>(smem);
 
-    const int slot = blockIdx.x / E, plane = blockIdx.x - slot * E;
-    const Rep R = reps[slot];
     if (i > R.N - 1) return;  // this replica's chain is complete
     const bool xslab = (i % 2 == 0);
     const bool finish = (i >= 1), cont = (i < R.N - 1);
@@ -278,21 +275,40 @@ __global__ __launch_bounds__((E / 2) * R2) void k_slab(const Rep* __restrict__ r
     }
 }
 
+// Workgroup (slot * E + plane, field): slot picks the replica descriptor.
+template <int E, int R1, int R2>
+__global__ __launch_bounds__((E / 2) * R2) void k_slab(const Rep* __restrict__ reps, const double2* __restrict__ tw,
+                                                     int i) {
+    const int slot = blockIdx.x / E, plane = blockIdx.x - slot * E;
+    const Rep R = reps[slot];
+    slab_pass<E, R1, R2>(R, plane, tw, i);
+}
+
+// One replica, its descriptor passed by value in the kernel arguments (no dependent
+// descriptor load before the first field loads).
+template <int E, int R1, int R2>
+__global__ __launch_bounds__((E / 2) * R2) void k_slab1(const Rep R, const double2* __restrict__ tw, int i) {
+    slab_pass<E, R1, R2>(R, blockIdx.x, tw, i);
+}
+
 bool slab_supported(int n0, int n1, int n2) {
     return n0 == n1 && n1 == n2 && (n0 == 32 || n0 == 48 || n0 == 64);
 }
 
-hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st) {
+hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st, const Rep* one) {
     const dim3 grid(nact * n, 2);
-    if (n == 64) {
-        hipLaunchKernelGGL((k_slab<64, 8, 8>), grid, dim3(32 * 8), 0, st, reps, tw, i);
-    } else if (n == 48) {
-        hipLaunchKernelGGL((k_slab<48, 12, 4>), grid, dim3(24 * 4), 0, st, reps, tw, i);
-    } else if (n == 32) {
-        hipLaunchKernelGGL((k_slab<32, 8, 4>), grid, dim3(16 * 4), 0, st, reps, tw, i);
-    } else {
-        return hipErrorInvalidValue;
-    }
+#define SLAB(E, R1, R2)                                                                                   \
+    do {                                                                                                  \
+        if (one && nact == 1)                                                                             \
+            hipLaunchKernelGGL((k_slab1<E, R1, R2>), dim3(E, 2), dim3((E / 2) * R2), 0, st, *one, tw, i); \
+        else                                                                                              \
+            hipLaunchKernelGGL((k_slab<E, R1, R2>), grid, dim3((E / 2) * R2), 0, st, reps, tw, i);        \
+    } while (0)
+    if (n == 64) SLAB(64, 8, 8);
+    else if (n == 48) SLAB(48, 12, 4);
+    else if (n == 32) SLAB(32, 8, 4);
+    else return hipErrorInvalidValue;
+#undef SLAB
     return hipGetLastError();
 }
 

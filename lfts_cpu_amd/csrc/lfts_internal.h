@@ -24,6 +24,7 @@ inline int blocks_for(long M) {
 
 enum { LINE_CONV = 0, LINE_FFT = 1 };
 
+
 // Device-side descriptor of one replica.
 struct Rep {
     double* w;          // 4M: W-, w+, phi-, phi+
@@ -77,9 +78,11 @@ hipError_t launch_line(const LineArgs& a, hipStream_t st);
 // ---- fused slab pass over replicas with a cubic mesh of edge n -----------------------
 // Step i = 0 is the prologue (x-slab pass on q_1); step i >= 1 finishes contour step i
 // (y-slab pass for odd i, x-slab pass for even i). `reps` holds nact descriptors of
-// replicas with edge n.
+// replicas with edge n; when nact == 1 and `one` (a host copy of that descriptor) is
+// given, the descriptor travels in the kernel arguments instead.
 bool slab_supported(int n0, int n1, int n2);
-hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st);
+hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st,
+                       const Rep* one = nullptr);
 
 // ---- batched element-wise and reduction kernels ----------------------------------------
 // reps: descriptors of the nact active replicas; grid (nb_max, nact).
