@@ -140,6 +140,49 @@ int lfts_batch_profile(lfts_batch* b, int enable);
 int lfts_batch_profile_read(lfts_batch* b, double* ms_chain, long long* n_passes, double* ms_pe,
                             long long* n_prop_evals);
 
+/* Derivatives of one propagator evaluation (no reference counterpart: the reference
+ * computes no derivatives). Each call first runs calc_concs at the context's current
+ * [W-|w+] (so [phi-|phi+] in the context are those of the current fields), then:
+ *   lfts_jvp: dphi = d(phi-, phi+) along the direction (dWm, dwp) (M doubles each, either
+ *             may be NULL for zero) -> 2*M doubles [dphi- | dphi+];
+ *   lfts_vjp: wbar = gradient of <u-, phi-> + <u+, phi+> with respect to (W-, w+), for
+ *             ubar = [u- | u+] (2*M) -> 2*M doubles [d/dW- | d/dw+].
+ * Both follow the chain of src/diblock.cc:30-60 and src/step.cc:35-65 exactly (including
+ * the Boltzmann factor of the chain ends for NA = 0 or NA = N). _dev variants take device
+ * pointers. */
+int lfts_jvp(lfts_ctx* ctx, const double* dWm, const double* dwp, double* dphi);
+int lfts_vjp(lfts_ctx* ctx, const double* ubar, double* wbar);
+int lfts_jvp_dev(lfts_ctx* ctx, const double* dWm_dev, const double* dwp_dev, double* dphi_dev);
+int lfts_vjp_dev(lfts_ctx* ctx, const double* ubar_dev, double* wbar_dev);
+
+/* Box derivative of H = -lnQ + (sum W-^2/XbN - sum w+)/M (src/lfts_simulation.cc:122-135)
+ * with respect to the box edges L[0..2] at fixed fields (H depends on the box through
+ * g(k) of src/step.cc:84-85 only): dHdL (3 doubles) = -d lnQ/dL. Also returns lnQ.
+ * Requires 1 <= NA <= N-1. */
+int lfts_box_derivative(lfts_ctx* ctx, double* dHdL, double* lnQ);
+
+/* Newton-Krylov saddle-point solve for w+ at fixed W-, beside anderson::mix
+ * (src/anderson.cc:24-93; no reference counterpart): inexact Newton on
+ * F(w+) = phi+(w+) - 1 with restarted GMRES over lfts_jvp products, preconditioned by the
+ * exact inverse response of the homogeneous melt, and backtracking. It stops when
+ * err = sqrt(sum F^2 / M) (the residual of src/anderson.cc:35-40) is <= tol. On return
+ * [phi-|phi+] in the context belong to the final w+. The mean of w+ is kept. */
+typedef struct lfts_nk_opts {
+    int maxNewton;     /* Newton iterations at most */
+    double tol;        /* residual tolerance */
+    int restart;       /* GMRES restart length, 1..30 (20 is typical) */
+    int maxInner;      /* tangent evaluations per Newton step at most */
+    double eta;        /* largest relative tolerance of a Newton step's linear solve */
+    int precondition;  /* 1: homogeneous-melt preconditioner, 0: none */
+} lfts_nk_opts;
+typedef struct lfts_nk_info {
+    int newton;        /* Newton iterations run */
+    int krylov;        /* tangent evaluations (lfts_jvp products) */
+    int prop_evals;    /* propagator evaluations (calc_concs) */
+    double err;        /* final residual */
+} lfts_nk_info;
+int lfts_saddle_nk(lfts_ctx* ctx, const lfts_nk_opts* opts, lfts_nk_info* info);
+
 /* Host random stream of the reference: std::mt19937_64 seeded by the caller (the
  * reference seeds 123456789, src/lfts_simulation.cc:77) feeding one persistent
  * std::normal_distribution<double>(0, sigma) (src/langevin.cc:13,16,28); uniforms use a

@@ -34,6 +34,26 @@ class Desc(ctypes.Structure):
     ]
 
 
+class NkOpts(ctypes.Structure):
+    _fields_ = [
+        ("maxNewton", ctypes.c_int),
+        ("tol", ctypes.c_double),
+        ("restart", ctypes.c_int),
+        ("maxInner", ctypes.c_int),
+        ("eta", ctypes.c_double),
+        ("precondition", ctypes.c_int),
+    ]
+
+
+class NkInfo(ctypes.Structure):
+    _fields_ = [
+        ("newton", ctypes.c_int),
+        ("krylov", ctypes.c_int),
+        ("prop_evals", ctypes.c_int),
+        ("err", ctypes.c_double),
+    ]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
@@ -67,6 +87,12 @@ SIGNATURES = {
     "lfts_counters": (ctypes.c_int, [_vp, _llp, _llp]),
     "lfts_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lfts_profile_read": (ctypes.c_int, [_vp, _dp, _llp, _dp, _llp]),
+    "lfts_jvp": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "lfts_vjp": (ctypes.c_int, [_vp, _dp, _dp]),
+    "lfts_jvp_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "lfts_vjp_dev": (ctypes.c_int, [_vp, _vp, _vp]),
+    "lfts_box_derivative": (ctypes.c_int, [_vp, _dp, _dp]),
+    "lfts_saddle_nk": (ctypes.c_int, [_vp, ctypes.POINTER(NkOpts), ctypes.POINTER(NkInfo)]),
     "lfts_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
     "lfts_batch_destroy": (ctypes.c_int, [_vp]),
     "lfts_batch_last_error": (ctypes.c_char_p, [_vp]),
@@ -223,6 +249,42 @@ class Context:
         self._chk(lib().lfts_step_fwd(self.h, _d(np.ascontiguousarray(q_in)), _d(out),
                                       _d(np.ascontiguousarray(hh)), int(i)))
         return out
+
+    # derivatives
+    def jvp(self, dWm=None, dwp=None) -> np.ndarray:
+        """[dphi- | dphi+] along (dW-, dw+) at the current fields."""
+        out = np.zeros(2 * self.M)
+        a = None if dWm is None else _d(np.ascontiguousarray(dWm, dtype=np.float64))
+        b = None if dwp is None else _d(np.ascontiguousarray(dwp, dtype=np.float64))
+        self._chk(lib().lfts_jvp(self.h, a, b, _d(out)))
+        return out
+
+    def vjp(self, ubar: np.ndarray) -> np.ndarray:
+        """[d/dW- | d/dw+] of <u-, phi-> + <u+, phi+> at the current fields."""
+        out = np.zeros(2 * self.M)
+        self._chk(lib().lfts_vjp(self.h, _d(np.ascontiguousarray(ubar, dtype=np.float64)), _d(out)))
+        return out
+
+    def jvp_dev(self, dWm_ptr, dwp_ptr, dphi_ptr):
+        self._chk(lib().lfts_jvp_dev(self.h, ctypes.c_void_p(dWm_ptr or None), ctypes.c_void_p(dwp_ptr or None),
+                                     ctypes.c_void_p(dphi_ptr)))
+
+    def vjp_dev(self, ubar_ptr, wbar_ptr):
+        self._chk(lib().lfts_vjp_dev(self.h, ctypes.c_void_p(ubar_ptr), ctypes.c_void_p(wbar_ptr)))
+
+    def box_derivative(self):
+        """(dH/dL as a length-3 array, lnQ) at fixed fields."""
+        out = np.zeros(3)
+        lnQ = ctypes.c_double()
+        self._chk(lib().lfts_box_derivative(self.h, _d(out), ctypes.byref(lnQ)))
+        return out, lnQ.value
+
+    def saddle_nk(self, tol: float = 1e-8, maxNewton: int = 30, restart: int = 20, maxInner: int = 200,
+                  eta: float = 0.1, precondition: bool = True) -> dict:
+        o = NkOpts(int(maxNewton), float(tol), int(restart), int(maxInner), float(eta), int(bool(precondition)))
+        info = NkInfo()
+        self._chk(lib().lfts_saddle_nk(self.h, ctypes.byref(o), ctypes.byref(info)))
+        return {"newton": info.newton, "krylov": info.krylov, "prop_evals": info.prop_evals, "err": info.err}
 
     def profile(self, enable: bool = True):
         self._chk(lib().lfts_profile(self.h, int(enable)))

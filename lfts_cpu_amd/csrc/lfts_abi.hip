@@ -20,79 +20,13 @@
 
 #include "lfts_gpu.h"
 #include "lfts_internal.h"
+#include "lfts_state.h"
 
 using lfts::Coef;
 using lfts::LineArgs;
 using lfts::Rep;
 
-namespace {
-
-struct AxisPlan {
-    int n = 1, nst = 0;
-    int radix[lfts::MAXST] = {0};
-    double2* tw = nullptr;    // exp(-2 pi i k/n)
-    double* gperm = nullptr;  // kernel at in-place DIF positions (line path)
-    int* perm = nullptr;      // DIF position of frequency k
-    double* gnat = nullptr;   // kernel in natural frequency order (slab path)
-    int T = 1, pitch = 2;
-};
-
-}  // namespace
-
-struct lfts_batch;
-
-struct lfts_ctx {
-    lfts_desc d{};
-    long M = 0, Mk = 0;
-    int N = 0, NA = 0, NB = 0, nhMax = 10;
-    AxisPlan ax[3];
-    bool fused = false;       // cubic 32/48/64 mesh: one slab launch per contour step
-    double* w = nullptr;      // 4M
-    double* h = nullptr;      // 2M (pair 1)
-    double* q = nullptr;      // (N-1) pairs
-    double* T0 = nullptr;     // 2M partial pair (slab path) / line-path scratch
-    double* T1 = nullptr;     // 2M partial pair (slab path)
-    double* Dh = nullptr;     // (nhMax+1) M
-    double* wh = nullptr;     // (nhMax+1) M
-    double* noise = nullptr;  // 2M: eta_prev | staging for a host eta_new
-    double* S = nullptr;      // Mk
-    double2* zk = nullptr;    // M, S(k) transform scratch (allocated on first sample)
-    double* test = nullptr;   // 8M, lfts_step_fwd scratch (allocated on first use)
-    int nsamples = 0;
-    long long n_pe = 0, n_cs = 0;
-    std::vector<double> DD, U, V, C;  // Anderson host state (src/anderson.cc:8-21)
-    lfts_batch* self = nullptr;       // set of one for the single-context entry points
-    std::string err;
-};
-
-struct lfts_batch {
-    std::vector<lfts_ctx*> ctx;
-    hipStream_t st = nullptr;
-    std::vector<Rep> reps;      // descriptors of all members (id = member index)
-    Rep* d_ract = nullptr;      // descriptors of the active members, in launch order
-    Rep* h_ract = nullptr;      // pinned staging of d_ract
-    int nb_max = 1;             // largest workgroup count among the active members
-    Coef* d_coef = nullptr;
-    Coef* h_coef = nullptr;     // pinned
-    double* d_part = nullptr;
-    double* d_scal = nullptr;
-    double* h_scal = nullptr;   // pinned
-    // the active set as launched: slab replicas sorted by edge, then line-path ones
-    std::vector<int> act;
-    struct Group { int edge, off, cnt; const double2* tw; };
-    std::vector<Group> groups;
-    std::vector<int> line_reps;
-    // live event timing
-    bool prof = false;
-    std::vector<hipEvent_t> ev_pool;
-    struct EvQuad { hipEvent_t pe0, ch0, ch1, pe1; long long npasses; int nrep; };
-    std::vector<EvQuad> ev_pending;
-    double prof_ms_chain = 0.0, prof_ms_pe = 0.0;
-    long long prof_n_pass = 0, prof_n_pe = 0;
-    std::string err;
-};
-
-namespace {
+namespace lfts_abi {
 
 int fail(lfts_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -105,19 +39,6 @@ int bfail(lfts_batch* b, int code, const std::string& msg) {
     }
     return code;
 }
-
-#define HIPCHK(ctx, expr)                                                                     \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return fail((ctx), LFTS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-#define BHIPCHK(b, expr)                                                                      \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return bfail((b), LFTS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 bool factorize(int n, std::vector<int>& r) {
     r.clear();
@@ -151,12 +72,14 @@ int build_axis(lfts_ctx* c, AxisPlan& p, int n, double L, int N) {
     }
     // Per-axis factor of g(k) = exp(-K^2/(6N))/M, K = 2 pi |k|/L (src/step.cc:84-85),
     // with the reference's frequency convention k in [-(n-1)/2, n/2] (src/step.cc:73-83).
-    std::vector<double> gperm(n), gnat(n);
+    // kap = (d g/dL)/g = (2 pi f)^2 / (3 N L^3), for the box derivative
+    std::vector<double> gperm(n), gnat(n), kap(n);
     std::vector<int> perm(n);
     for (int k = 0; k < n; k++) {
         const int f = (k <= n / 2) ? k : k - n;
         const double K = 2.0 * M_PI * (double)f / L;
         gnat[k] = std::exp(-K * K / (6.0 * N)) / (double)n;
+        kap[k] = K * K / (3.0 * N * L);
         perm[k] = dif_pos(k, rad, 0, n);
         gperm[perm[k]] = gnat[k];
     }
@@ -164,6 +87,8 @@ int build_axis(lfts_ctx* c, AxisPlan& p, int n, double L, int N) {
     HIPCHK(c, hipMalloc(&p.gperm, sizeof(double) * n));
     HIPCHK(c, hipMalloc(&p.perm, sizeof(int) * n));
     HIPCHK(c, hipMalloc(&p.gnat, sizeof(double) * n));
+    HIPCHK(c, hipMalloc(&p.kap, sizeof(double) * n));
+    HIPCHK(c, hipMemcpy(p.kap, kap.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(p.tw, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(p.gperm, gperm.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(p.perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
@@ -544,7 +469,9 @@ int batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
     return LFTS_OK;
 }
 
-}  // namespace
+}  // namespace lfts_abi
+
+using namespace lfts_abi;
 
 extern "C" {
 
@@ -616,10 +543,13 @@ int lfts_destroy(lfts_ctx* c) {
         (void)hipFree(p.gperm);
         (void)hipFree(p.perm);
         (void)hipFree(p.gnat);
+        (void)hipFree(p.kap);
     }
     for (void* p : {(void*)c->w, (void*)c->h, (void*)c->q, (void*)c->T0, (void*)c->T1, (void*)c->Dh, (void*)c->wh,
-                    (void*)c->noise, (void*)c->S, (void*)c->zk, (void*)c->test})
+                    (void*)c->noise, (void*)c->S, (void*)c->zk, (void*)c->test, (void*)c->dq, (void*)c->dbuf,
+                    (void*)c->acc, (void*)c->dred, (void*)c->kry, (void*)c->pmul})
         (void)hipFree(p);
+    if (c->hred) (void)hipHostFree(c->hred);
     delete c;
     return LFTS_OK;
 }

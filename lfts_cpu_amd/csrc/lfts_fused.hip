@@ -179,8 +179,13 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 
 }  // namespace
 
-template <int E, int R1, int R2>
-__device__ __forceinline__ void slab_pass(const Rep& R, int plane, const double2* __restrict__ tw, int i) {
+// One slab pass of one replica (plane `plane`, field blockIdx.y), addresses in `io`.
+// MODE selects the finishing rule of pair i+1 from u = (C x)(r):
+//   CHAIN_PLAIN    out = h u                   (src/step.cc:53-64)
+//   CHAIN_TANGENT  out = h u + aux src         (directional derivative of the chain)
+//   CHAIN_ADJOINT  out = u + aux src, stored; the chain continues with h out
+template <int E, int R1, int R2, int MODE>
+__device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const double2* __restrict__ tw) {
     static_assert(R1 * R2 == E && R1 % R2 == 0, "unsupported factorisation");
     constexpr int H = E / 2;           // complex lines per plane
     constexpr int U = R1 / R2;
@@ -189,20 +194,18 @@ __device__ __forceinline__ void slab_pass(const Rep& R, int plane, const double2
     constexpr int XBYTES = H * XP * 16;
     constexpr int RBYTES = E * RP * 8;
     constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
+    constexpr bool EXTRA = MODE != CHAIN_PLAIN;
     __shared__ __align__(16) unsigned char smem[BYTES];
     double2* xbuf = reinterpret_cast<double2*>(smem);
     double* rp = reinterpret_cast<double*>(smem);
 
-    if (i > R.N - 1) return;  // this replica's chain is complete
-    const bool xslab = (i % 2 == 0);
-    const bool finish = (i >= 1), cont = (i < R.N - 1);
-    const long M = R.M;
+    const bool xslab = io.xslab != 0;
+    const bool finish = io.finish != 0, cont = io.cont != 0;
+    const long M = io.M;
     const long plane_stride = xslab ? (long)E * E : E;
     const long Sa = xslab ? E : (long)E * E;
-    const double* Tin = (i == 0) ? R.h : ((i & 1) ? R.T0 : R.T1);
-    double* Tout = (i & 1) ? R.T1 : R.T0;
-    const double* ga = xslab ? R.g[1] : R.g[0];
-    const double* gb = R.g[2];
+    const double* ga = xslab ? io.g[1] : io.g[0];
+    const double* gb = io.g[2];
 
     const int t = threadIdx.x;
     const int f = blockIdx.y;
@@ -212,14 +215,23 @@ __device__ __forceinline__ void slab_pass(const Rep& R, int plane, const double2
     const int r = t / R2, jb = t % R2; // layout B: z-lines, row pair (r, r + H), b = jb + R2 k
 
     // everything this thread reads is issued up front: the partial pair, the Boltzmann
-    // factor for its field at this bead (src/step.cc:53-64), twiddles and kernel values
-    double2 z[R1], hz[R1];
-    const double* hsel = f ? ((i < R.NB) ? R.h + M : R.h) : ((i < R.NA) ? R.h : R.h + M);
+    // factor of its field at this bead (src/step.cc:53-64), twiddles and kernel values
+    const double* Tin = io.tin + (long)f * M;
+    const double* hsel = f ? io.h[1] : io.h[0];
+    const double* asel = f ? io.aux[1] : io.aux[0];
+    const double* ssel = f ? io.src[1] : io.src[0];
+    double2 z[R1], hz[R1], az[EXTRA ? R1 : 1], sz[EXTRA ? R1 : 1];
 #pragma unroll
     for (int k = 0; k < R1; k++) {
         const long p = pbase + (long)(j + R2 * k) * Sa + c;
-        z[k] = make_double2(Tin[(long)f * M + p], Tin[(long)f * M + p + H]);
-        if (finish) hz[k] = make_double2(hsel[p], hsel[p + H]);
+        z[k] = make_double2(Tin[p], Tin[p + H]);
+        if (finish) {
+            hz[k] = make_double2(hsel[p], hsel[p + H]);
+            if constexpr (EXTRA) {
+                az[k] = make_double2(asel[p], asel[p + H]);
+                sz[k] = make_double2(ssel[p], ssel[p + H]);
+            }
+        }
     }
     double2 twA[R1], twB[R1];
     double gA[R1], gB[R1];
@@ -238,14 +250,26 @@ __device__ __forceinline__ void slab_pass(const Rep& R, int plane, const double2
 
     if (finish) {
         conv_line<R1, R2>(z, xbuf + c * XP, j, twA, gA);
-        double* qout = R.q + 2L * (i - 1) * M;  // pair i+1
+        double* qout = io.qout;
 #pragma unroll
         for (int k = 0; k < R1; k++) {
             const long p = base + (long)(j + R2 * k) * Sa + c;
-            z[k].x *= hz[k].x;
-            z[k].y *= hz[k].y;
+            if constexpr (MODE == CHAIN_PLAIN) {
+                z[k].x *= hz[k].x;
+                z[k].y *= hz[k].y;
+            } else if constexpr (MODE == CHAIN_TANGENT) {
+                z[k].x = z[k].x * hz[k].x + az[k].x * sz[k].x;
+                z[k].y = z[k].y * hz[k].y + az[k].y * sz[k].y;
+            } else {
+                z[k].x += az[k].x * sz[k].x;
+                z[k].y += az[k].y * sz[k].y;
+            }
             qout[p] = z[k].x;
             qout[p + H] = z[k].y;
+            if constexpr (MODE == CHAIN_ADJOINT) {
+                z[k].x *= hz[k].x;
+                z[k].y *= hz[k].y;
+            }
         }
     }
     if (!cont) return;
@@ -267,6 +291,7 @@ __device__ __forceinline__ void slab_pass(const Rep& R, int plane, const double2
         w[k] = make_double2(rp[r * RP + b], rp[(r + H) * RP + b]);
     }
     conv_line<R1, R2>(w, xbuf + r * XP, jb, twB, gB);
+    double* Tout = io.tout;
 #pragma unroll
     for (int k = 0; k < R1; k++) {
         const long p = base + (long)r * Sa + jb + R2 * k;
@@ -275,20 +300,23 @@ __device__ __forceinline__ void slab_pass(const Rep& R, int plane, const double2
     }
 }
 
-// Workgroup (slot * E + plane, field): slot picks the replica descriptor.
+// Plain chains of a replica set. Workgroup (slot * E + plane, field): slot picks the
+// replica descriptor.
 template <int E, int R1, int R2>
 __global__ __launch_bounds__((E / 2) * R2) void k_slab(const Rep* __restrict__ reps, const double2* __restrict__ tw,
                                                      int i) {
     const int slot = blockIdx.x / E, plane = blockIdx.x - slot * E;
-    const Rep R = reps[slot];
-    slab_pass<E, R1, R2>(R, plane, tw, i);
+    const Rep& R = reps[slot];  // fields read in place (a private copy would live in scratch)
+    if (i > R.N - 1) return;    // this replica's chain is complete
+    const SlabIO io = plain_io(R, i);
+    slab_pass<E, R1, R2, CHAIN_PLAIN>(io, plane, tw);
 }
 
-// One replica, its descriptor passed by value in the kernel arguments (no dependent
-// descriptor load before the first field loads).
-template <int E, int R1, int R2>
-__global__ __launch_bounds__((E / 2) * R2) void k_slab1(const Rep R, const double2* __restrict__ tw, int i) {
-    slab_pass<E, R1, R2>(R, blockIdx.x, tw, i);
+// One replica, its pass addresses in the kernel arguments (no dependent descriptor load
+// before the first field loads).
+template <int E, int R1, int R2, int MODE>
+__global__ __launch_bounds__((E / 2) * R2) void k_slab_io(const SlabIO io, const double2* __restrict__ tw) {
+    slab_pass<E, R1, R2, MODE>(io, blockIdx.x, tw);
 }
 
 bool slab_supported(int n0, int n1, int n2) {
@@ -296,19 +324,33 @@ bool slab_supported(int n0, int n1, int n2) {
 }
 
 hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st, const Rep* one) {
+    if (one && nact == 1) {
+        if (i > one->N - 1) return hipSuccess;
+        return launch_slab_io(n, CHAIN_PLAIN, plain_io(*one, i), tw, st);
+    }
     const dim3 grid(nact * n, 2);
-#define SLAB(E, R1, R2)                                                                                   \
-    do {                                                                                                  \
-        if (one && nact == 1)                                                                             \
-            hipLaunchKernelGGL((k_slab1<E, R1, R2>), dim3(E, 2), dim3((E / 2) * R2), 0, st, *one, tw, i); \
-        else                                                                                              \
-            hipLaunchKernelGGL((k_slab<E, R1, R2>), grid, dim3((E / 2) * R2), 0, st, reps, tw, i);        \
-    } while (0)
+#define SLAB(E, R1, R2) hipLaunchKernelGGL((k_slab<E, R1, R2>), grid, dim3((E / 2) * R2), 0, st, reps, tw, i)
     if (n == 64) SLAB(64, 8, 8);
     else if (n == 48) SLAB(48, 12, 4);
     else if (n == 32) SLAB(32, 8, 4);
     else return hipErrorInvalidValue;
 #undef SLAB
+    return hipGetLastError();
+}
+
+hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st) {
+#define SLABIO(E, R1, R2)                                                                                      \
+    do {                                                                                                       \
+        const dim3 grid(E, 2), block((E / 2) * R2);                                                            \
+        if (mode == CHAIN_PLAIN) hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN>), grid, block, 0, st, io, tw);     \
+        else if (mode == CHAIN_TANGENT) hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_TANGENT>), grid, block, 0, st, io, tw); \
+        else hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_ADJOINT>), grid, block, 0, st, io, tw);                 \
+    } while (0)
+    if (n == 64) SLABIO(64, 8, 8);
+    else if (n == 48) SLABIO(48, 12, 4);
+    else if (n == 32) SLABIO(32, 8, 4);
+    else return hipErrorInvalidValue;
+#undef SLABIO
     return hipGetLastError();
 }
 

@@ -22,7 +22,9 @@ inline int blocks_for(long M) {
     return (int)(nb < 1 ? 1 : (nb > MAX_NB ? MAX_NB : nb));
 }
 
-enum { LINE_CONV = 0, LINE_FFT = 1 };
+enum { LINE_CONV = 0, LINE_FFT = 1, LINE_IFFT = 2, LINE_CMUL = 3 };
+// kinds of propagator chain: the plain chain, its tangent, its adjoint
+enum { CHAIN_PLAIN = 0, CHAIN_TANGENT = 1, CHAIN_ADJOINT = 2 };
 
 
 // Device-side descriptor of one replica.
@@ -72,10 +74,58 @@ struct LineArgs {
     const double2* inC;
     double2* outC;
     double* Sacc;
+    // LINE_CONV with hA/hB (the finishing pass of a chain step), chain kind `chain`:
+    //   CHAIN_TANGENT  out = h u + aux src;  CHAIN_ADJOINT  out = u + aux src, out2 = h out
+    int chain;
+    const double* auxA;
+    const double* auxB;
+    const double* srcA;
+    const double* srcB;
+    double* out2A;
+    double* out2B;
+    // LINE_IFFT: real part to outR instead of outC; LINE_CMUL: per-point multiplier in
+    // the line's transform order, mul[line * n + e]
+    double* outR;
+    const double* mul;
 };
 hipError_t launch_line(const LineArgs& a, hipStream_t st);
 
 // ---- fused slab pass over replicas with a cubic mesh of edge n -----------------------
+
+// Addresses of one slab pass of one replica. Pass i finishes pair i+1 from the partial
+// convolution `tin` (i >= 1) and leaves the partial convolution of the next step in
+// `tout`; pass 0 (the prologue) starts from the chain's first pair in `tin`. Every
+// pointer addresses pair layout [field 0 | field 1] except h/aux/src, which point at the
+// M values used for field 0 (q) and field 1 (q+) separately.
+struct SlabIO {
+    const double* tin;
+    double* tout;
+    double* qout;          // pair i+1 (finishing passes)
+    const double* h[2];    // Boltzmann factor per field at this step
+    const double* aux[2];  // tangent: e per field; adjoint: alpha per field
+    const double* src[2];  // the stored values aux multiplies, per field
+    const double* g[3];    // per-axis kernels, natural frequency order
+    long M;
+    int finish, cont, xslab;
+};
+
+// Pass i of a replica's plain chain: pair 1 = h, pair i+1 = h_sel C pair_i
+// (src/diblock.cc:40-43, src/step.cc:53-64); T0/T1 alternate as partial pairs.
+__host__ __device__ inline SlabIO plain_io(const Rep& R, int i) {
+    SlabIO io{};
+    const long M = R.M;
+    io.tin = (i == 0) ? R.h : ((i & 1) ? R.T0 : R.T1);
+    io.tout = (i & 1) ? R.T1 : R.T0;
+    io.qout = (i >= 1) ? R.q + 2L * (i - 1) * M : nullptr;
+    io.h[0] = (i < R.NA) ? R.h : R.h + M;
+    io.h[1] = (i < R.NB) ? R.h + M : R.h;
+    for (int d = 0; d < 3; d++) io.g[d] = R.g[d];
+    io.M = M;
+    io.finish = i >= 1;
+    io.cont = i < R.N - 1;
+    io.xslab = (i % 2 == 0);
+    return io;
+}
 // Step i = 0 is the prologue (x-slab pass on q_1); step i >= 1 finishes contour step i
 // (y-slab pass for odd i, x-slab pass for even i). `reps` holds nact descriptors of
 // replicas with edge n; when nact == 1 and `one` (a host copy of that descriptor) is
@@ -83,6 +133,8 @@ hipError_t launch_line(const LineArgs& a, hipStream_t st);
 bool slab_supported(int n0, int n1, int n2);
 hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st,
                        const Rep* one = nullptr);
+// One pass of one replica's chain of kind `mode` (CHAIN_*), grid (n planes, 2 fields).
+hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st);
 
 // ---- batched element-wise and reduction kernels ----------------------------------------
 // reps: descriptors of the nact active replicas; grid (nb_max, nact).

@@ -130,8 +130,12 @@ __device__ __forceinline__ long line_base(const LineArgs& a, long line) {
 
 // Generic line kernel: T lines of length n along `axis` per workgroup.
 //  mode LINE_CONV: real pair in (inA,inB) -> FFT -> *gperm -> IFFT -> [*h] -> real pair out
-//  mode LINE_FFT : complex (or real) in -> forward FFT -> natural order complex out, or,
-//                  on the last axis, |X|^2 accumulated into the half spectrum Sacc.
+//  mode LINE_FFT : complex, real, or real pair (inA + i inB) in -> forward FFT -> natural
+//                  order complex out, or, on the last axis, |X|^2 accumulated into the
+//                  half spectrum Sacc.
+//  mode LINE_IFFT: natural-order complex in -> inverse FFT -> complex out, or its real
+//                  part (outR).
+//  mode LINE_CMUL: complex in -> FFT -> *mul (per point) -> IFFT -> complex out.
 __global__ __launch_bounds__(256) void k_line(LineArgs a) {
     extern __shared__ double2 lds[];
     const int n = a.n, T = a.T, pitch = a.pitch;
@@ -145,29 +149,39 @@ __global__ __launch_bounds__(256) void k_line(LineArgs a) {
         double2 v = make_double2(0.0, 0.0);
         if (line < a.nlines) {
             const long p = line_base(a, line) + (long)e * a.stride;
-            if (a.mode == LINE_CONV) v = make_double2(a.inA[p], a.inB[p]);
+            if (a.inA) v = make_double2(a.inA[p], a.inB[p]);
             else if (a.inR) v = make_double2(a.inR[p], 0.0);
             else v = a.inC[p];
         }
-        lds[t * pitch + e] = v;
+        // the inverse-only mode places frequency e where the forward pass would leave it
+        lds[t * pitch + (a.mode == LINE_IFFT ? a.perm[e] : e)] = v;
     }
     __syncthreads();
 
     int B = n;
-    for (int s = 0; s < a.nst; s++) {
-        run_stage<false>(a.radix[s], lds, pitch, T, n, B, a.tw);
-        __syncthreads();
-        B /= a.radix[s];
+    if (a.mode != LINE_IFFT) {
+        for (int s = 0; s < a.nst; s++) {
+            run_stage<false>(a.radix[s], lds, pitch, T, n, B, a.tw);
+            __syncthreads();
+            B /= a.radix[s];
+        }
+    } else {
+        B = 1;
     }
 
-    if (a.mode == LINE_CONV) {
-        for (int f = threadIdx.x; f < T * n; f += blockDim.x) {
-            const int t = f / n, e = f - t * n;
-            const double g = a.gperm[e];
-            double2 v = lds[t * pitch + e];
-            lds[t * pitch + e] = make_double2(v.x * g, v.y * g);
+    if (a.mode != LINE_FFT) {
+        if (a.mode == LINE_CONV || a.mode == LINE_CMUL) {
+            for (int f = threadIdx.x; f < T * n; f += blockDim.x) {
+                const int t = f / n, e = f - t * n;
+                const long line = line0 + t;
+                double g = 0.0;
+                if (a.mode == LINE_CONV) g = a.gperm[e];
+                else if (line < a.nlines) g = a.mul[line * n + e];
+                double2 v = lds[t * pitch + e];
+                lds[t * pitch + e] = make_double2(v.x * g, v.y * g);
+            }
+            __syncthreads();
         }
-        __syncthreads();
         for (int s = a.nst - 1; s >= 0; s--) {
             B *= a.radix[s];
             run_stage<true>(a.radix[s], lds, pitch, T, n, B, a.tw);
@@ -180,8 +194,26 @@ __global__ __launch_bounds__(256) void k_line(LineArgs a) {
             if (line >= a.nlines) continue;
             const long p = line_base(a, line) + (long)e * a.stride;
             double2 v = lds[t * pitch + e];
+            if (a.mode != LINE_CONV) {
+                if (a.outR) a.outR[p] = v.x;
+                else a.outC[p] = v;
+                continue;
+            }
             double x = v.x, y = v.y;
-            if (a.hA) { x *= a.hA[p]; y *= a.hB[p]; }
+            if (a.hA) {
+                if (a.chain == CHAIN_TANGENT) {
+                    x = x * a.hA[p] + a.auxA[p] * a.srcA[p];
+                    y = y * a.hB[p] + a.auxB[p] * a.srcB[p];
+                } else if (a.chain == CHAIN_ADJOINT) {
+                    x += a.auxA[p] * a.srcA[p];
+                    y += a.auxB[p] * a.srcB[p];
+                    a.out2A[p] = x * a.hA[p];
+                    a.out2B[p] = y * a.hB[p];
+                } else {
+                    x *= a.hA[p];
+                    y *= a.hB[p];
+                }
+            }
             a.outA[p] = x;
             a.outB[p] = y;
         }

@@ -1,0 +1,119 @@
+// Host-side state of contexts and replica sets, shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "lfts_gpu.h"
+#include "lfts_internal.h"
+
+namespace lfts_abi {
+
+struct AxisPlan {
+    int n = 1, nst = 0;
+    int radix[lfts::MAXST] = {0};
+    double2* tw = nullptr;    // exp(-2 pi i k/n)
+    double* gperm = nullptr;  // kernel at in-place DIF positions (line path)
+    int* perm = nullptr;      // DIF position of frequency k
+    double* gnat = nullptr;   // kernel in natural frequency order (slab path)
+    double* kap = nullptr;    // d g/dL / g in natural frequency order: (2 pi f)^2 / (3 N L^3)
+    int T = 1, pitch = 2;
+};
+
+}  // namespace lfts_abi
+
+struct lfts_batch;
+
+struct lfts_ctx {
+    lfts_desc d{};
+    long M = 0, Mk = 0;
+    int N = 0, NA = 0, NB = 0, nhMax = 10;
+    lfts_abi::AxisPlan ax[3];
+    bool fused = false;       // cubic 32/48/64 mesh: one slab launch per contour step
+    double* w = nullptr;      // 4M
+    double* h = nullptr;      // 2M (pair 1)
+    double* q = nullptr;      // (N-1) pairs
+    double* T0 = nullptr;     // 2M partial pair (slab path) / line-path scratch
+    double* T1 = nullptr;     // 2M partial pair (slab path)
+    double* Dh = nullptr;     // (nhMax+1) M
+    double* wh = nullptr;     // (nhMax+1) M
+    double* noise = nullptr;  // 2M: eta_prev | staging for a host eta_new
+    double* S = nullptr;      // Mk
+    double2* zk = nullptr;    // M, transform scratch (allocated on first use)
+    double* test = nullptr;   // 8M, lfts_step_fwd scratch (allocated on first use)
+    // derivative chains (allocated on first use)
+    double* dq = nullptr;     // (N-1) pairs of a tangent or adjoint chain (pairs 2..N)
+    double* dbuf = nullptr;   // 10M: d1 (2M) | d2 (2M) | aux (2M) | io (4M)
+    double* acc = nullptr;    // M, spectral accumulator of the box derivative
+    double* dred = nullptr;   // reduction partials (32 x MAX_NB) and 64 results
+    double* hred = nullptr;   // pinned host copy of the results
+    // Newton-Krylov workspace (allocated on first use)
+    double* kry = nullptr;    // (kry_m + 1) M Krylov basis, then 5M work vectors
+    int kry_m = 0;
+    double* pmul = nullptr;   // M, preconditioner multiplier in the x-line transform order
+    int nsamples = 0;
+    long long n_pe = 0, n_cs = 0, n_tan = 0, n_adj = 0;
+    std::vector<double> DD, U, V, C;  // Anderson host state (src/anderson.cc:8-21)
+    lfts_batch* self = nullptr;       // set of one for the single-context entry points
+    std::string err;
+};
+
+struct lfts_batch {
+    std::vector<lfts_ctx*> ctx;
+    hipStream_t st = nullptr;
+    std::vector<lfts::Rep> reps;  // descriptors of all members (id = member index)
+    lfts::Rep* d_ract = nullptr;  // descriptors of the active members, in launch order
+    lfts::Rep* h_ract = nullptr;  // pinned staging of d_ract
+    int nb_max = 1;               // largest workgroup count among the active members
+    lfts::Coef* d_coef = nullptr;
+    lfts::Coef* h_coef = nullptr;  // pinned
+    double* d_part = nullptr;
+    double* d_scal = nullptr;
+    double* h_scal = nullptr;  // pinned
+    // the active set as launched: slab replicas sorted by edge, then line-path ones
+    std::vector<int> act;
+    struct Group { int edge, off, cnt; const double2* tw; };
+    std::vector<Group> groups;
+    std::vector<int> line_reps;
+    // live event timing
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct EvQuad { hipEvent_t pe0, ch0, ch1, pe1; long long npasses; int nrep; };
+    std::vector<EvQuad> ev_pending;
+    double prof_ms_chain = 0.0, prof_ms_pe = 0.0;
+    long long prof_n_pass = 0, prof_n_pe = 0;
+    std::string err;
+};
+
+#define HIPCHK(ctx, expr)                                                                                \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return lfts_abi::fail((ctx), LFTS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define BHIPCHK(b, expr)                                                                                  \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return lfts_abi::bfail((b), LFTS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace lfts_abi {
+
+int fail(lfts_ctx* c, int code, const std::string& msg);
+int bfail(lfts_batch* b, int code, const std::string& msg);
+lfts::LineArgs line_args(const lfts_ctx* c, int axis, int mode);
+double* pair_ptr(lfts_ctx* c, int i);
+int line_contour_step(lfts_ctx* c, const double* in, double* out, const double* h, int i, double* tmp,
+                      hipStream_t st);
+int set_active(lfts_batch* b, const std::vector<int>& ids);
+int calc_concs_async(lfts_batch* b);
+int read_scal(lfts_batch* b);
+int sync_batch(lfts_batch* b);
+std::vector<int> all_ids(const lfts_batch* b);
+void lu_solve(int n, const double* A_in, const double* b, double* x);
+bool factorize(int n, std::vector<int>& r);
+int dif_pos(int k, const std::vector<int>& rad, size_t s, int B);
+
+}  // namespace lfts_abi

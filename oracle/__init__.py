@@ -48,6 +48,9 @@ def lib():
             "orc_calc_concs": (ctypes.c_double, [_vp, _dp]),
             "orc_get_pair": (None, [_vp, ctypes.c_int, _dp]),
             "orc_step_fwd": (None, [_vp, _dp, _dp, _dp, ctypes.c_int]),
+            "orc_jvp": (None, [_vp, _dp, _dp, _dp, _dp]),
+            "orc_vjp": (None, [_vp, _dp, _dp, _dp]),
+            "orc_dlnQ_dL": (ctypes.c_double, [_vp, _dp, _dp]),
             "orc_am_create": (_vp, [ctypes.c_int, ctypes.c_int]),
             "orc_am_destroy": (None, [_vp]),
             "orc_am_mix": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double, _dp]),
@@ -134,6 +137,28 @@ class Diblock:
         out = np.zeros(2 * self.M)
         lib().orc_get_pair(self.h, int(i), _d(out))
         return out
+
+    def jvp(self, w: np.ndarray, dwm, dwp) -> np.ndarray:
+        """Directional derivative [dphi- | dphi+] of calc_concs at w along (dW-, dw+)."""
+        out = np.zeros(2 * self.M)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        a = None if dwm is None else _d(np.ascontiguousarray(dwm, dtype=np.float64))
+        b = None if dwp is None else _d(np.ascontiguousarray(dwp, dtype=np.float64))
+        lib().orc_jvp(self.h, _d(w), a, b, _d(out))
+        return out
+
+    def vjp(self, w: np.ndarray, ubar: np.ndarray) -> np.ndarray:
+        """Gradient [dL/dW- | dL/dw+] of L = <u-, phi-> + <u+, phi+> at w."""
+        out = np.zeros(2 * self.M)
+        lib().orc_vjp(self.h, _d(np.ascontiguousarray(w, dtype=np.float64)),
+                      _d(np.ascontiguousarray(ubar, dtype=np.float64)), _d(out))
+        return out
+
+    def dlnQ_dL(self, w: np.ndarray):
+        """(ln Q, d lnQ/dL) at fixed fields."""
+        out = np.zeros(3)
+        lnQ = lib().orc_dlnQ_dL(self.h, _d(np.ascontiguousarray(w, dtype=np.float64)), _d(out))
+        return lnQ, out
 
     def step_fwd(self, q_in: np.ndarray, h: np.ndarray, i: int) -> np.ndarray:
         out = np.zeros(2 * self.M)

@@ -31,6 +31,8 @@
 //                  mt19937_64 known answer and a Python restatement of the polar method
 //   strFunc        single cosine mode (|w_k|^2 = (A M/2)^2), bins vs numpy, weights
 //   getH           closed form for uniform fields
+//   jvp, vjp       central finite differences of calc_concs; <u, J v> = <J^T u, v>
+//   dlnQ/dL        central finite differences of calc_concs over the box edges
 //   z_inf          mpmath at 50 digits
 // =====================================================================================
 #include <algorithm>
@@ -263,6 +265,178 @@ static double diblock_calc_concs(orc_diblock& d, double* w) {
         phiB[r] = pA + pB;
     }
     return log(Q);
+}
+
+// -------------------------------------------------------------------------------------
+// Derivatives of calc_concs (no reference counterpart; pinned by central finite
+// differences of diblock_calc_concs and by the identity <u, J v> = <J^T u, v>).
+//
+// Chain of the reference (src/diblock.cc:40-43, src/step.cc:53-64): q_1 = hA,
+// q_j = f_j C q_{j-1}; q+_N = hB, q+_j = p_j C q+_{j+1}, with the Boltzmann factor of
+// bead j being f_j (q chain: A for j = 1, else A iff j <= NA) and p_j (q+ chain: B for
+// j = N, else B iff j > NA). phiA/phiB sum beads j <= NA / j > NA (src/diblock.cc:51-53).
+// -------------------------------------------------------------------------------------
+static inline bool fq_is_A(int j, int NA) { return j == 1 || j <= NA; }
+static inline bool pq_is_A(int j, int N, int NA) { return j != N && j <= NA; }
+
+// (oa, ob) = (C a, C b) with the step's kernel g (src/step.cc:44-50)
+static void conv2(orc_step& s, const double* a, const double* b, double* oa, double* ob) {
+    const int M = s.M;
+    for (int r = 0; r < M; r++) s.z[r] = cplx(a[r], b[r]);
+    fft_3d(s.P, s.m, s.z.data(), -1);
+    for (int r = 0; r < M; r++) s.z[r] *= s.gfull[r];
+    fft_3d(s.P, s.m, s.z.data(), +1);
+    for (int r = 0; r < M; r++) { oa[r] = s.z[r].real(); ob[r] = s.z[r].imag(); }
+}
+
+// Directional derivative of (phi-, phi+) at w along (dW-, dw+) (either may be null).
+static void diblock_jvp(orc_diblock& d, const double* w_in, const double* dwm, const double* dwp, double* dphi) {
+    const int M = d.M, N = d.N, NA = d.NA, NB = N - NA;
+    std::vector<double> w(w_in, w_in + 4L * M);
+    diblock_calc_concs(d, w.data());
+    const double* hA = d.h.data();
+    const double* hB = d.h.data() + M;
+    double Q = 0.0;
+    for (int r = 0; r < M; r++) Q += d.q1(N)[r];
+    Q /= M;
+    std::vector<double> eA(M), eB(M);
+    for (int r = 0; r < M; r++) {
+        const double u = dwm ? dwm[r] : 0.0, v = dwp ? dwp[r] : 0.0;
+        eA[r] = -(u + v) / N;
+        eB[r] = -(v - u) / N;
+    }
+    // dp[i] = [dq_i | dq+_{N+1-i}], same layout as the pairs
+    std::vector<double> dp(2L * (N + 1) * M, 0.0);
+    auto D = [&](int i) { return dp.data() + 2L * i * M; };
+    for (int r = 0; r < M; r++) { D(1)[r] = eA[r] * hA[r]; D(1)[r + M] = eB[r] * hB[r]; }
+    std::vector<double> ca(M), cb(M);
+    for (int i = 1; i < N; i++) {
+        conv2(d.st, D(i), D(i) + M, ca.data(), cb.data());
+        const double* hq = (i < NA) ? hA : hB;
+        const double* eq = (i < NA) ? eA.data() : eB.data();
+        const double* hqd = (i < NB) ? hB : hA;
+        const double* eqd = (i < NB) ? eB.data() : eA.data();
+        const double* P = d.q1(i + 1);
+        for (int r = 0; r < M; r++) {
+            D(i + 1)[r] = ca[r] * hq[r] + eq[r] * P[r];
+            D(i + 1)[r + M] = cb[r] * hqd[r] + eqd[r] * P[r + M];
+        }
+    }
+    double dQ = 0.0;
+    for (int r = 0; r < M; r++) dQ += D(N)[r];
+    dQ /= M;
+    for (int r = 0; r < M; r++) {
+        double SA = 0, SB = 0, dSA = 0, dSB = 0;
+        for (int i = 1; i <= N; i++) {
+            const double q = d.q1(i)[r], qd = d.q2(i)[r];
+            const double dq = D(i)[r], dqd = D(N + 1 - i)[r + M];
+            if (i <= NA) { SA += q * qd; dSA += dq * qd + q * dqd; }
+            else { SB += q * qd; dSB += dq * qd + q * dqd; }
+        }
+        const double pA = SA / (N * Q * hA[r]), pB = SB / (N * Q * hB[r]);
+        const double dpA = dSA / (N * Q * hA[r]) - pA * (dQ / Q + eA[r]);
+        const double dpB = dSB / (N * Q * hB[r]) - pB * (dQ / Q + eB[r]);
+        dphi[r] = dpA - dpB;
+        dphi[r + M] = dpA + dpB;
+    }
+}
+
+// Gradient (dW-, dw+) of L = <u-, phi-> + <u+, phi+> at w, for ubar = [u- | u+].
+static void diblock_vjp(orc_diblock& d, const double* w_in, const double* ubar, double* wbar) {
+    const int M = d.M, N = d.N, NA = d.NA;
+    std::vector<double> w(w_in, w_in + 4L * M);
+    diblock_calc_concs(d, w.data());
+    const double* hA = d.h.data();
+    const double* hB = d.h.data() + M;
+    double Q = 0.0;
+    for (int r = 0; r < M; r++) Q += d.q1(N)[r];
+    Q /= M;
+    std::vector<double> phA(M, 0.0), phB(M, 0.0), uA(M), uB(M), aA(M), aB(M);
+    double Sdot = 0.0;
+    for (int r = 0; r < M; r++) {
+        double SA = 0, SB = 0;
+        for (int i = 1; i <= N; i++) {
+            const double v = d.q1(i)[r] * d.q2(i)[r];
+            if (i <= NA) SA += v; else SB += v;
+        }
+        phA[r] = SA / (N * Q * hA[r]);
+        phB[r] = SB / (N * Q * hB[r]);
+        uA[r] = ubar[r + M] + ubar[r];
+        uB[r] = ubar[r + M] - ubar[r];
+        Sdot += uA[r] * phA[r] + uB[r] * phB[r];
+        aA[r] = uA[r] / (N * Q * hA[r]);
+        aB[r] = uB[r] / (N * Q * hB[r]);
+    }
+    const double Qbar = -Sdot / Q;
+    auto alpha = [&](int j) { return (j <= NA) ? aA.data() : aB.data(); };
+    auto fq = [&](int j) { return fq_is_A(j, NA) ? hA : hB; };
+    auto pq = [&](int j) { return pq_is_A(j, N, NA) ? hA : hB; };
+    // qb[j] = dL/dq_j (j = N..1), qdb[j] = dL/dq+_j (j = 1..N)
+    std::vector<double> qb((size_t)(N + 1) * M), qdb((size_t)(N + 1) * M), ta(M), tb(M), ca(M), cb(M);
+    auto QB = [&](int j) { return qb.data() + (size_t)j * M; };
+    auto QDB = [&](int j) { return qdb.data() + (size_t)j * M; };
+    for (int r = 0; r < M; r++) {
+        QB(N)[r] = alpha(N)[r] * d.q2(N)[r] + Qbar / M;
+        QDB(1)[r] = alpha(1)[r] * d.q1(1)[r];
+    }
+    for (int m = 1; m < N; m++) {
+        const int j = N - m, jd = m + 1;  // qb[j] from qb[j+1]; qdb[jd] from qdb[jd-1]
+        for (int r = 0; r < M; r++) {
+            ta[r] = fq(j + 1)[r] * QB(j + 1)[r];
+            tb[r] = pq(jd - 1)[r] * QDB(jd - 1)[r];
+        }
+        conv2(d.st, ta.data(), tb.data(), ca.data(), cb.data());
+        for (int r = 0; r < M; r++) {
+            QB(j)[r] = alpha(j)[r] * d.q2(j)[r] + ca[r];
+            QDB(jd)[r] = alpha(jd)[r] * d.q1(jd)[r] + cb[r];
+        }
+    }
+    for (int r = 0; r < M; r++) {
+        double GA = -uA[r] * phA[r], GB = -uB[r] * phB[r];
+        for (int j = 1; j <= N; j++) {
+            const double a = QB(j)[r] * d.q1(j)[r];
+            if (fq_is_A(j, NA)) GA += a; else GB += a;
+            const double b = QDB(j)[r] * d.q2(j)[r];
+            if (pq_is_A(j, N, NA)) GA += b; else GB += b;
+        }
+        wbar[r] = -(GA - GB) / N;
+        wbar[r + M] = -(GA + GB) / N;
+    }
+}
+
+// d lnQ / d L_d at fixed fields through g(k) = exp(-K^2/(6N))/M (src/step.cc:84-85):
+// dQ/dL_d = (1/M) sum_{i<N} sum_k g(k) kap_d(k) Re(X_i(k) conj(Y_{i+1}(k))), X_i = F q_i,
+// Y_i = F q+_i, kap_d = (2 pi k_d)^2 / (3 N L_d^3). Valid when the q+ chain is the
+// adjoint of the q chain (1 <= NA <= N-1). Returns ln Q.
+static double diblock_dlnQ_dL(orc_diblock& d, const double* w_in, double* out) {
+    const int M = d.M, N = d.N;
+    std::vector<double> w(w_in, w_in + 4L * M);
+    const double lnQ = diblock_calc_concs(d, w.data());
+    const double Q = exp(lnQ);
+    const int* m = d.st.m;
+    std::vector<double> acc(M, 0.0);
+    std::vector<cplx> X(M), Y(M);
+    for (int i = 1; i < N; i++) {
+        for (int r = 0; r < M; r++) { X[r] = cplx(d.q1(i)[r], 0.0); Y[r] = cplx(d.q2(i + 1)[r], 0.0); }
+        fft_3d(d.st.P, m, X.data(), -1);
+        fft_3d(d.st.P, m, Y.data(), -1);
+        for (int r = 0; r < M; r++) acc[r] += (X[r] * std::conj(Y[r])).real();
+    }
+    for (int dd = 0; dd < 3; dd++) {
+        double s = 0.0;
+        for (int K0 = 0; K0 < m[0]; K0++)
+            for (int K1 = 0; K1 < m[1]; K1++)
+                for (int K2 = 0; K2 < m[2]; K2++) {
+                    const int Kd = (dd == 0) ? K0 : (dd == 1 ? K1 : K2);
+                    const int f = (Kd <= m[dd] / 2) ? Kd : Kd - m[dd];
+                    const double L = d.st.L[dd];
+                    const double kap = (2 * M_PI * f) * (2 * M_PI * f) / (3.0 * N * L * L * L);
+                    const long k = ((long)K0 * m[1] + K1) * m[2] + K2;
+                    s += d.st.gfull[k] * kap * acc[k];
+                }
+        out[dd] = s / M / Q;
+    }
+    return lnQ;
 }
 
 // -------------------------------------------------------------------------------------
@@ -579,6 +753,16 @@ double orc_calc_concs(void* h, double* w) { return diblock_calc_concs(*static_ca
 void orc_get_pair(void* h, int i, double* out) {
     orc_diblock* d = static_cast<orc_diblock*>(h);
     std::memcpy(out, d->q1(i), sizeof(double) * 2 * d->M);
+}
+
+void orc_jvp(void* h, const double* w, const double* dwm, const double* dwp, double* dphi) {
+    diblock_jvp(*static_cast<orc_diblock*>(h), w, dwm, dwp, dphi);
+}
+void orc_vjp(void* h, const double* w, const double* ubar, double* wbar) {
+    diblock_vjp(*static_cast<orc_diblock*>(h), w, ubar, wbar);
+}
+double orc_dlnQ_dL(void* h, const double* w, double* out) {
+    return diblock_dlnQ_dL(*static_cast<orc_diblock*>(h), w, out);
 }
 
 void orc_step_fwd(void* h, const double* q_in, double* q_out, const double* hh, int i) {
