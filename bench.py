@@ -66,7 +66,7 @@ def measured_traffic(mesh):
     """HBM bytes per contour-step launch from the committed FETCH_SIZE/WRITE_SIZE
     summary (two rocprofv3 --pmc runs, scaled by a copy of known size); None when no
     summary exists for this mesh."""
-    path = os.path.join(ROOT, "profiles", f"k_slab_{mesh}cubed_N90_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"k_slab_io_{mesh}cubed_N90_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -181,6 +181,67 @@ def batched_leg(steps, warmup, L_mod, torch, quick=False):
     }
 
 
+def adjoint_leg(ctx, eta_dev, XbN, dt, M, torch, tols=(1e-4, 1e-12)):
+    """Newton-Krylov against Anderson on identical W-: from the state after the timed
+    steps, one more W- update (as at the start of every Langevin step), then both
+    solvers from the same fields to the same residual, at the production tolerance and
+    at a tight one (where the two w+ must agree). Also times the tangent and adjoint
+    products (each includes its forward evaluation) and the box derivative."""
+    ctx.langevin_step_dev(eta_dev, XbN, dt)
+    w_start = ctx.get_fields(2)
+    ctx.saddle_nk(tol=1e-2)  # one-time set-up (workspace, preconditioner table) outside the timing
+    rows = []
+    for tol in tols:
+        ctx.set_fields(w_start)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k, err_a = ctx.anderson_mix(5000, tol)
+        t_and = time.perf_counter() - t0
+        w_and = ctx.get_fields(2)
+        ctx.set_fields(w_start)
+        t0 = time.perf_counter()
+        info = ctx.saddle_nk(tol=tol)
+        t_nk = time.perf_counter() - t0
+        w_nk = ctx.get_fields(2)
+        rows.append({
+            "tol": tol,
+            "anderson_prop_evals": int(k) - 1,
+            "anderson_err": err_a,
+            "anderson_ms": 1e3 * t_and,
+            "nk_newton": info["newton"],
+            "nk_prop_evals": info["prop_evals"],
+            "nk_tangent_evals": info["krylov"],
+            "nk_chain_evals": info["prop_evals"] + info["krylov"],
+            "nk_err": info["err"],
+            "nk_ms": 1e3 * t_nk,
+            "max_abs_dwplus": float(np.abs(w_nk[M:] - w_and[M:]).max()),
+        })
+    dev = torch.empty(6 * M, dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cpu").manual_seed(5)
+    dev[:3 * M].copy_(torch.randn(3 * M, generator=g, dtype=torch.float64))
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.jvp_dev(dev.data_ptr(), dev.data_ptr() + 8 * M, dev.data_ptr() + 8 * 3 * M)
+    t_jvp = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.vjp_dev(dev.data_ptr() + 8 * M, dev.data_ptr() + 8 * 3 * M)
+    t_vjp = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    ctx.box_derivative()
+    t_box = time.perf_counter() - t0
+    return {
+        "workload": "64^3 N=90 state after the timed steps plus one W- update; Anderson (10 histories) and "
+                    "Newton-Krylov (GMRES(20), homogeneous-melt preconditioner) from the same fields",
+        "solves": rows,
+        "jvp_with_forward_ms": 1e3 * t_jvp,
+        "vjp_with_forward_ms": 1e3 * t_vjp,
+        "box_derivative_ms": 1e3 * t_box,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +252,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=3, help="timed steps of the replica-set leg (0 skips it)")
     ap.add_argument("--batched-quick", action="store_true", help="8 members instead of 64 (smoke runs)")
     ap.add_argument("--only-batched", action="store_true", help="run only the replica-set leg (profiling)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the Newton-Krylov / derivative leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -223,8 +285,8 @@ def main():
     log(f"initial mix: k={k0} err={err0:.3e} ({time.perf_counter() - t0:.2f} s)")
 
     nsteps = args.warmup + args.steps
-    eta = torch.empty((nsteps, M), dtype=torch.float64, device="cuda")
-    for s in range(nsteps):
+    eta = torch.empty((nsteps + 1, M), dtype=torch.float64, device="cuda")
+    for s in range(nsteps + 1):
         eta[s].copy_(torch.from_numpy(rng.normals(M)))
     torch.cuda.synchronize()
 
@@ -260,6 +322,11 @@ def main():
             cpu = cpu_baseline(ctx.get_fields(4), params, m, Lb, N, NA)
         except Exception as e:  # the baseline is reported, not required
             log(f"cpu baseline failed: {e!r}")
+
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(ctx, eta[nsteps].data_ptr(), XbN, dt, M, torch)
+        log(f"adjoint leg: {adjoint}")
 
     batched = None
     if args.batched_steps > 0:
@@ -303,6 +370,7 @@ def main():
         "propagator_eval_ms": ms_pe / max(n_pe, 1),
         "cpu_baseline": cpu,
         "batched": batched,
+        "newton_krylov": adjoint,
     }
     print(json.dumps(out), flush=True)
 

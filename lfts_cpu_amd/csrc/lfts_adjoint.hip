@@ -530,11 +530,18 @@ int ensure_precond(lfts_ctx* c) {
                 double v = 0.0;
                 if (K0 || K1 || K2) {
                     const double G = G1[0][K0] * G1[1][K1] * G1[2][K2];
-                    double s = 0.0, part = 0.0, gt = 1.0;
-                    for (int j = 1; j <= N; j++) {
-                        part += gt;
-                        gt *= G;
-                        s += part;
+                    double s = 0.0;
+                    if (1.0 - G > 1e-2) {
+                        // sum_{j=1..N} (1 - G^j)/(1 - G) = (N - G (1 - G^N)/(1 - G)) / (1 - G)
+                        const double u = 1.0 - G;
+                        s = (N - G * (1.0 - std::pow(G, N)) / u) / u;
+                    } else {
+                        double part = 0.0, gt = 1.0;
+                        for (int j = 1; j <= N; j++) {
+                            part += gt;
+                            gt *= G;
+                            s += part;
+                        }
                     }
                     const double lam = -(1.0 / N) * ((2.0 / N) * s - 1.0);
                     v = 1.0 / (lam * (double)M);
