@@ -293,15 +293,26 @@ int calc_concs_async(lfts_batch* b) {
     }
     BHIPCHK(b, lfts::launch_prepare_h(b->d_ract, nact, b->nb_max, b->st));
     if (b->prof) BHIPCHK(b, hipEventRecord(ev.ch0, b->st));
-    int Nmax = 0;
-    for (int id : b->act) Nmax = std::max(Nmax, b->ctx[id]->N);
     long long npasses = 0;
-    // slab passes: i = 0 prologue, i = 1..N-1 contour steps (src/diblock.cc:43)
-    for (int i = 0; i < Nmax; i++)
-        for (const auto& g : b->groups) {
-            BHIPCHK(b, lfts::launch_slab(b->d_ract + g.off, g.cnt, g.edge, i, g.tw, b->st, b->h_ract + g.off));
+    // slab passes: i = 0 prologue, i = 1..N-1 contour steps (src/diblock.cc:43). With
+    // more than one mesh size each group's chain runs on its own stream.
+    const int ng = (int)b->groups.size();
+    const bool fork = ng > 1;
+    if (fork) {
+        BHIPCHK(b, hipEventRecord(b->ev_fork, b->st));
+        for (int g = 0; g < ng; g++) BHIPCHK(b, hipStreamWaitEvent(b->gst[g], b->ev_fork, 0));
+    }
+    for (int g = 0; g < ng; g++) {
+        const auto& G = b->groups[g];
+        hipStream_t s = fork ? b->gst[g] : b->st;
+        int Ng = 0;
+        for (int k = 0; k < G.cnt; k++) Ng = std::max(Ng, b->ctx[b->act[G.off + k]]->N);
+        for (int i = 0; i < Ng; i++) {
+            BHIPCHK(b, lfts::launch_slab(b->d_ract + G.off, G.cnt, G.edge, i, G.tw, s, b->h_ract + G.off));
             if (i > 0) npasses++;
         }
+    }
+    // line-path replicas run on the set's stream meanwhile
     for (int id : b->line_reps) {
         lfts_ctx* c = b->ctx[id];
         for (int i = 1; i < c->N; i++) {
@@ -310,6 +321,11 @@ int calc_concs_async(lfts_batch* b) {
             npasses++;
         }
     }
+    if (fork)
+        for (int g = 0; g < ng; g++) {
+            BHIPCHK(b, hipEventRecord(b->ev_join[g], b->gst[g]));
+            BHIPCHK(b, hipStreamWaitEvent(b->st, b->ev_join[g], 0));
+        }
     if (b->prof) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
     BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_part, b->st));
     BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, 1, 1, b->d_scal, 0, b->st));
@@ -451,6 +467,13 @@ int batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
         if ((expr) != hipSuccess) return cleanup(LFTS_EHIP); \
     } while (0)
     BCREATE(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+    if (n > 1) {
+        for (int g = 0; g < 3; g++) {
+            BCREATE(hipStreamCreateWithFlags(&b->gst[g], hipStreamNonBlocking));
+            BCREATE(hipEventCreateWithFlags(&b->ev_join[g], hipEventDisableTiming));
+        }
+        BCREATE(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    }
     b->reps.resize(n);
     for (int i = 0; i < n; i++) {
         b->reps[i] = make_rep(ctxs[i]);
@@ -786,6 +809,14 @@ int lfts_batch_destroy(lfts_batch* b) {
     for (auto& q : b->ev_pending)
         for (hipEvent_t e : {q.pe0, q.ch0, q.ch1, q.pe1}) b->ev_pool.push_back(e);
     for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);
+    for (int g = 0; g < 3; g++) {
+        if (b->gst[g]) {
+            (void)hipStreamSynchronize(b->gst[g]);
+            (void)hipStreamDestroy(b->gst[g]);
+        }
+        if (b->ev_join[g]) (void)hipEventDestroy(b->ev_join[g]);
+    }
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     for (void* p : {(void*)b->d_ract, (void*)b->d_coef, (void*)b->d_part, (void*)b->d_scal}) (void)hipFree(p);
     for (void* p : {(void*)b->h_ract, (void*)b->h_coef, (void*)b->h_scal})
         if (p) (void)hipHostFree(p);

@@ -76,6 +76,11 @@ struct lfts_batch {
     struct Group { int edge, off, cnt; const double2* tw; };
     std::vector<Group> groups;
     std::vector<int> line_reps;
+    // one stream per slab group, so the groups of a contour step overlap on the GPU;
+    // forked from and joined back to `st` around each propagator chain
+    hipStream_t gst[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
     // live event timing
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
