@@ -72,6 +72,9 @@ int lfts_langevin_init(lfts_ctx* ctx, const double* eta_prev);
  * eta_new (M doubles) drawn by the caller: host pointer / device pointer variants. */
 int lfts_langevin_step(lfts_ctx* ctx, const double* eta_new, double XbN, double dt);
 int lfts_langevin_step_dev(lfts_ctx* ctx, const double* eta_new_dev, double XbN, double dt);
+/* The noise langevin keeps between steps (src/langevin.cc:34-36), M host doubles, for a
+ * checkpoint; lfts_langevin_init restores it. */
+int lfts_langevin_get_prev(lfts_ctx* ctx, double* eta_prev);
 
 /* One whole Langevin step of lfts_simulation::equilibrate()/statistics()
  * (src/lfts_simulation.cc:67-87): step_wm with device noise, mix(maxIter, errTol), then
@@ -84,6 +87,9 @@ int lfts_langevin_full_step_dev(lfts_ctx* ctx, const double* eta_new_dev, double
 int lfts_sk_sample(lfts_ctx* ctx);
 int lfts_sk_get(lfts_ctx* ctx, double* S, int* nsamples);
 int lfts_sk_reset(lfts_ctx* ctx);
+/* Restores S (Mk doubles) and the sample count, for a restart from a checkpoint (no
+ * reference counterpart: the reference restarts from its 6-digit field files only). */
+int lfts_sk_set(lfts_ctx* ctx, const double* S, int nsamples);
 
 /* strFunc::calcK + the stable sort of the constructor (src/strFunc.cc:27-31,80-103),
  * host-side: K (Mk doubles), wt (Mk ints), P (Mk ints). No context needed. */
@@ -192,6 +198,12 @@ int lfts_rng_create(unsigned long long seed, double sigma, lfts_rng** out);
 int lfts_rng_normals(lfts_rng* rng, double* out, long long n);
 int lfts_rng_uniforms(lfts_rng* rng, double* out, long long n);
 int lfts_rng_destroy(lfts_rng* rng);
+/* The full state of the stream (engine, and the normal distribution with the value the
+ * polar method keeps for its next call) as text: lfts_rng_save with buf = NULL sets *len
+ * to the size needed; lfts_rng_load restores it, so a restarted run continues the very
+ * same stream (the reference re-seeds on every start, src/lfts_simulation.cc:23). */
+int lfts_rng_save(lfts_rng* rng, char* buf, long long cap, long long* len);
+int lfts_rng_load(lfts_rng* rng, const char* buf, long long len);
 
 /* lfts_params::z_inf_discrete (src/lfts_params.cc:122-142). */
 double lfts_z_inf(const double* L, const int* m, int N, double Nbar, int tmax);

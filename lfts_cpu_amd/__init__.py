@@ -14,6 +14,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "liblfts_gpu.so")
 DRIVER_PATH = os.path.join(PKG, "bin", "lfts-gpu")
+VTK_PATH = os.path.join(PKG, "bin", "lfts-vtk")
 
 LFTS_OK, LFTS_EINVAL, LFTS_EHIP, LFTS_ESTATE, LFTS_ENUM = range(5)
 
@@ -108,6 +109,10 @@ SIGNATURES = {
     "lfts_rng_normals": (ctypes.c_int, [_vp, _dp, ctypes.c_longlong]),
     "lfts_rng_uniforms": (ctypes.c_int, [_vp, _dp, ctypes.c_longlong]),
     "lfts_rng_destroy": (ctypes.c_int, [_vp]),
+    "lfts_rng_save": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_longlong, _llp]),
+    "lfts_rng_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_longlong]),
+    "lfts_sk_set": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "lfts_langevin_get_prev": (ctypes.c_int, [_vp, _dp]),
     "lfts_z_inf": (ctypes.c_double, [_dp, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
 }
 
@@ -233,6 +238,14 @@ class Context:
 
     def sk_reset(self):
         self._chk(lib().lfts_sk_reset(self.h))
+
+    def sk_set(self, S: np.ndarray, nsamples: int):
+        self._chk(lib().lfts_sk_set(self.h, _d(np.ascontiguousarray(S, dtype=np.float64)), int(nsamples)))
+
+    def langevin_get_prev(self) -> np.ndarray:
+        out = np.zeros(self.M)
+        self._chk(lib().lfts_langevin_get_prev(self.h, _d(out)))
+        return out
 
     def hamiltonian(self, XbN: float) -> float:
         H = ctypes.c_double()
@@ -392,6 +405,20 @@ class Rng:
         out = np.zeros(int(n))
         lib().lfts_rng_normals(self.h, _d(out), int(n))
         return out
+
+    def save_state(self) -> bytes:
+        n = ctypes.c_longlong()
+        lib().lfts_rng_save(self.h, None, 0, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value)
+        rc = lib().lfts_rng_save(self.h, buf, n.value, ctypes.byref(n))
+        if rc != LFTS_OK:
+            raise LftsError(rc, "lfts_rng_save failed")
+        return buf.raw[:n.value]
+
+    def load_state(self, state: bytes):
+        rc = lib().lfts_rng_load(self.h, state, len(state))
+        if rc != LFTS_OK:
+            raise LftsError(rc, "lfts_rng_load failed")
 
     def uniforms(self, n: int) -> np.ndarray:
         out = np.zeros(int(n))

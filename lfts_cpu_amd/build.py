@@ -3,6 +3,7 @@
 Outputs:
   lfts_cpu_amd/lib/liblfts_gpu.so   C-ABI library (kernels + host control)
   lfts_cpu_amd/bin/lfts-gpu         drop-in driver for `lfts-cpu <inputFile>`
+  lfts_cpu_amd/bin/lfts-vtk         field file -> VTK (the reference's tools/make_vtk.sh)
 """
 from __future__ import annotations
 
@@ -18,6 +19,7 @@ LIB_DIR = os.path.join(PKG, "lib")
 BIN_DIR = os.path.join(PKG, "bin")
 LIB = os.path.join(LIB_DIR, "liblfts_gpu.so")
 DRIVER = os.path.join(BIN_DIR, "lfts-gpu")
+VTK_TOOL = os.path.join(BIN_DIR, "lfts-vtk")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # FP64 throughout; device code may contract a*b+c into FMA (about 1 ulp from the
@@ -57,9 +59,19 @@ def build_driver() -> str:
     return DRIVER
 
 
+def build_vtk_tool() -> str:
+    """`lfts-vtk`: the field-file to VTK converter (host only, no GPU library)."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    tmp = VTK_TOOL + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-o", tmp, os.path.join(HOST, "make_vtk.cc")])
+    os.replace(tmp, VTK_TOOL)
+    return VTK_TOOL
+
+
 def build_all() -> None:
     build_library()
     build_driver()
+    build_vtk_tool()
 
 
 if __name__ == "__main__":

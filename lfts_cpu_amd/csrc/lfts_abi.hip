@@ -15,6 +15,7 @@
 #include <fstream>
 #include <numeric>
 #include <random>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -665,6 +666,21 @@ int lfts_sk_sample(lfts_ctx* c) {
     return LFTS_OK;
 }
 
+int lfts_sk_set(lfts_ctx* c, const double* S, int nsamples) {
+    if (!c || !S || nsamples < 0) return fail(c, LFTS_EINVAL, "sk_set: bad argument");
+    HIPCHK(c, hipMemcpyAsync(c->S, S, sizeof(double) * c->Mk, hipMemcpyHostToDevice, c->self->st));
+    HIPCHK(c, hipStreamSynchronize(c->self->st));
+    c->nsamples = nsamples;
+    return LFTS_OK;
+}
+
+int lfts_langevin_get_prev(lfts_ctx* c, double* eta_prev) {
+    if (!c || !eta_prev) return fail(c, LFTS_EINVAL, "langevin_get_prev: null argument");
+    HIPCHK(c, hipMemcpyAsync(eta_prev, c->noise, sizeof(double) * c->M, hipMemcpyDeviceToHost, c->self->st));
+    HIPCHK(c, hipStreamSynchronize(c->self->st));
+    return LFTS_OK;
+}
+
 int lfts_sk_get(lfts_ctx* c, double* S, int* nsamples) {
     if (!c) return LFTS_EINVAL;
     if (S) {
@@ -911,6 +927,32 @@ int lfts_rng_uniforms(lfts_rng* r, double* out, long long n) {
     if (!r || (!out && n > 0)) return LFTS_EINVAL;
     std::uniform_real_distribution<double> u(0.0, 1.0);
     for (long long i = 0; i < n; i++) out[i] = u(r->eng);
+    return LFTS_OK;
+}
+
+// The engine and the distribution (including the second normal that the polar method
+// keeps for the next call) as text, via the standard library's stream operators.
+int lfts_rng_save(lfts_rng* r, char* buf, long long cap, long long* len) {
+    if (!r || !len) return LFTS_EINVAL;
+    std::ostringstream os;
+    os << r->eng << ' ' << r->nd;
+    const std::string t = os.str();
+    *len = (long long)t.size();
+    if (!buf) return LFTS_OK;
+    if (cap < *len) return LFTS_EINVAL;
+    std::memcpy(buf, t.data(), t.size());
+    return LFTS_OK;
+}
+
+int lfts_rng_load(lfts_rng* r, const char* buf, long long len) {
+    if (!r || !buf || len <= 0) return LFTS_EINVAL;
+    std::istringstream is(std::string(buf, (size_t)len));
+    std::mt19937_64 eng;
+    std::normal_distribution<double> nd;
+    is >> eng >> nd;
+    if (is.fail()) return LFTS_EINVAL;
+    r->eng = eng;
+    r->nd = nd;
     return LFTS_OK;
 }
 

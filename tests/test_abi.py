@@ -130,3 +130,19 @@ def test_driver_rejects_zero_sample_period(tmp_path):
     f.write_text("10 5 12.0 50 0.5 0\n4 4 4 2.0 2.0 2.0\n2 2 0 1 0\n")
     r = subprocess.run([L.DRIVER_PATH, str(f)], capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode == 1 and "ERROR" in r.stdout
+
+
+def test_rng_state_round_trip_continues_the_stream():
+    # an odd number of draws leaves the polar method's second normal pending
+    for n0 in (0, 1, 7, 1000):
+        r = L.Rng(123456789, 2.5)
+        r.normals(n0)
+        state = r.save_state()
+        a = r.normals(101)
+        u = r.uniforms(5)
+        s = L.Rng(1, 1.0)
+        s.load_state(state)
+        assert np.array_equal(s.normals(101), a)
+        assert np.array_equal(s.uniforms(5), u)
+    with pytest.raises(L.LftsError):
+        L.Rng(1, 1.0).load_state(b"not a state")

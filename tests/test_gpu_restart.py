@@ -1,0 +1,80 @@
+"""Lossless checkpoint/restart of the `lfts-gpu` driver: a run stopped with a checkpoint
+and restarted from it prints the same lnQ lines and writes byte-identical files as the
+uninterrupted run (the reference's own resume path re-seeds its noise and reloads
+6-digit fields, so it cannot do this). Also the --vtk export."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import lfts_cpu_amd as L
+
+pytestmark = pytest.mark.gpu
+
+HEADER = "16 8 14.0 50 0.5 1\n12 12 12 3.0 3.0 3.0\n4 4 2 2 1\n"
+
+
+def _input(tmp_path):
+    M = 12 ** 3
+    rng = np.random.default_rng(31)
+    w0 = np.concatenate([rng.normal(0, 0.1, M), rng.normal(0, 0.1, M)])
+    p = tmp_path / "input"
+    with open(p, "w") as f:
+        f.write(HEADER)
+        for v in w0:
+            f.write("%.17g\n" % v)
+    return p
+
+
+def _run(args, cwd):
+    print("driver", args, flush=True)
+    r = subprocess.run([L.DRIVER_PATH, *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def _lnq(out):
+    return [ln for ln in out.splitlines() if ln.startswith("lnQ = ")]
+
+
+@pytest.mark.parametrize("stop", [3, 6])  # inside equilibration / inside statistics
+def test_stop_and_restart_is_identical_to_one_run(gpu, tmp_path, stop):
+    inp = _input(tmp_path)
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    out_a = _run([inp], a)
+    out_b1 = _run([inp, "--stop-after", stop, "--checkpoint", "ck"], b)
+    assert "Stopped after %d Langevin steps" % stop in out_b1
+    out_b2 = _run([inp, "--restart", "ck", "--checkpoint", "ck2"], b)
+    assert _lnq(out_b1) + _lnq(out_b2) == _lnq(out_a)
+    assert out_b2.strip().splitlines()[-1] == out_a.strip().splitlines()[-1]  # H
+    files_a = sorted(os.listdir(a))
+    files_b = sorted(f for f in os.listdir(b) if not f.startswith("ck"))
+    assert files_b == files_a
+    for f in files_a:
+        assert (a / f).read_bytes() == (b / f).read_bytes(), f
+
+
+def test_periodic_checkpoints_and_a_corrupt_one(gpu, tmp_path):
+    inp = _input(tmp_path)
+    _run([inp, "--checkpoint-every", 2, "--checkpoint", "ck"], tmp_path)
+    assert (tmp_path / "ck").exists() and not (tmp_path / "ck.tmp").exists()
+    data = bytearray((tmp_path / "ck").read_bytes())
+    data[100] ^= 1
+    (tmp_path / "bad").write_bytes(bytes(data))
+    r = subprocess.run([L.DRIVER_PATH, str(inp), "--restart", "bad"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 1 and "cannot read the checkpoint" in r.stdout
+
+
+def test_vtk_export_matches_the_converter(gpu, tmp_path):
+    inp = _input(tmp_path)
+    _run([inp, "--vtk"], tmp_path)
+    for f in ("phi_0", "phi_eq_2", "phi_st_4"):
+        got = (tmp_path / (f + ".vtk")).read_bytes()
+        os.rename(tmp_path / (f + ".vtk"), tmp_path / (f + ".vtk.driver"))
+        r = subprocess.run([L.VTK_PATH, f, "12", "12", "12"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0
+        assert (tmp_path / (f + ".vtk")).read_bytes() == got, f
