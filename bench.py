@@ -154,6 +154,7 @@ def batched_leg(steps, warmup, L_mod, torch, quick=False):
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     ms_chain, n_pass, ms_pe, n_pe = B.profile_read()
+    tuning = B.tuning()
     B.close()
     # bytes moved per the contour-step accounting (48 M per member step)
     cs_bytes = sum(48.0 * mb["M"] * (90 - 1) * (int(k) - 1 + 1) for ks in kb for mb, k in zip(members, ks))
@@ -178,6 +179,7 @@ def batched_leg(steps, warmup, L_mod, torch, quick=False):
         "mean_n_AM": float(np.mean([np.mean(k) - 1 for k in kb])),
         "contour_chain_algorithmic_GBps": cs_bytes / (ms_chain * 1e-3) / 1e9 if ms_chain > 0 else None,
         "chain_frac_of_8TBps": cs_bytes / (ms_chain * 1e-3) / 8e12 if ms_chain > 0 else None,
+        "slab_tuning": tuning.strip().splitlines(),
     }
 
 
@@ -306,6 +308,7 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_chain, n_cs, ms_pe, n_pe = ctx.profile_read()
     ctx.profile(False)
+    tuning_plain = ctx.tuning().strip()
     log(f"timed {args.steps} steps: {elapsed:.3f} s, n_AM={nAM}, lnQ={lnq:.10g}")
 
     steps_per_s = args.steps / elapsed
@@ -363,6 +366,7 @@ def main():
             "traffic": measured_traffic(args.mesh),
             "kernel": "contour step (48*M algorithmic bytes), HIP events on the context stream",
             "contour_step_us": cs_ms * 1e3,
+            "slab_tuning": tuning_plain,
         },
         "step_algorithmic_GBps": step_gbps,
         "step_roofline_frac": step_gbps / HBM_PEAK_GBPS,

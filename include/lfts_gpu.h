@@ -142,6 +142,13 @@ int lfts_batch_langevin_step_dev(lfts_batch* b, const double* const* eta_new_dev
                                  const double* dt);
 int lfts_batch_full_step_dev(lfts_batch* b, const double* const* eta_new_dev, const double* XbN, const double* dt,
                              int maxIter, double errTol, double* lnQ, int* k_ret);
+/* Autotuner report (no reference counterpart): the contour-step kernel shape chosen for
+ * each mesh size and launch size, with the measured time per pass of every shape. The
+ * shapes differ in where the twiddle/kernel tables live (registers or LDS) and in how
+ * many fields a workgroup serves; all give bit-identical results. Set LFTS_SLAB_VARIANT
+ * to 0..3 to fix the shape instead. The text stays valid until the next call on the set. */
+const char* lfts_batch_tuning(const lfts_batch* b);
+const char* lfts_tuning(const lfts_ctx* ctx);
 int lfts_batch_profile(lfts_batch* b, int enable);
 int lfts_batch_profile_read(lfts_batch* b, double* ms_chain, long long* n_passes, double* ms_pe,
                             long long* n_prop_evals);

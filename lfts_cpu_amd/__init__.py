@@ -103,6 +103,8 @@ SIGNATURES = {
     "lfts_batch_langevin_step_dev": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _dp, _dp]),
     "lfts_batch_full_step_dev": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _dp, _dp, ctypes.c_int, ctypes.c_double,
                                                 _dp, _ip]),
+    "lfts_batch_tuning": (ctypes.c_char_p, [_vp]),
+    "lfts_tuning": (ctypes.c_char_p, [_vp]),
     "lfts_batch_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lfts_batch_profile_read": (ctypes.c_int, [_vp, _dp, _llp, _dp, _llp]),
     "lfts_rng_create": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_double, ctypes.POINTER(_vp)]),
@@ -309,6 +311,9 @@ class Context:
         self._chk(lib().lfts_profile_read(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
         return a.value, b.value, c.value, d.value
 
+    def tuning(self) -> str:
+        return lib().lfts_tuning(self.h).decode()
+
     def counters(self):
         a = ctypes.c_longlong()
         b = ctypes.c_longlong()
@@ -374,6 +379,9 @@ class Batch:
         self._chk(lib().lfts_batch_full_step_dev(self.h, self._ptrs(eta_ptrs), _d(XbN), _d(dt), int(maxIter),
                                                  float(tol), _d(lnq), _i(k)))
         return lnq, k
+
+    def tuning(self) -> str:
+        return lib().lfts_batch_tuning(self.h).decode()
 
     def profile(self, enable: bool = True):
         self._chk(lib().lfts_batch_profile(self.h, int(enable)))

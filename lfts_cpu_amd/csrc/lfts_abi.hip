@@ -281,6 +281,56 @@ int read_scal(lfts_batch* b) {
 
 double scal(const lfts_batch* b, int id, int k) { return b->h_scal[(size_t)id * lfts::SSTRIDE + k]; }
 
+// The slab kernel variant (*variant) for `cnt` replicas of edge `edge` (descriptors
+// d_reps, host copies h_reps), chosen once per set and size class by timing each variant on contour-step passes 1
+// and 2 after a prologue pass (their outputs are rewritten by the next chain). The
+// environment variable LFTS_SLAB_VARIANT = 0..3 fixes the choice instead.
+int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep* h_reps, const double2* tw,
+                 hipStream_t s, int* variant) {
+    // launch sizes are tuned in power-of-two classes (the active set shrinks one member
+    // at a time as members converge)
+    int cls = 1;
+    while (cls < cnt) cls *= 2;
+    const auto key = std::make_pair(edge, cls);
+    const auto it = b->tuned.find(key);
+    if (it != b->tuned.end()) {
+        *variant = it->second;
+        return LFTS_OK;
+    }
+    int best = 0;
+    const char* env = std::getenv("LFTS_SLAB_VARIANT");
+    char line[160];
+    if (env && *env >= '0' && *env < '0' + lfts::SLAB_VARIANTS && env[1] == 0) {
+        best = *env - '0';
+        std::snprintf(line, sizeof line, "edge %d x %d: variant %d (fixed)\n", edge, cls, best);
+    } else {
+        for (hipEvent_t& e : b->tune_ev)
+            if (!e) BHIPCHK(b, hipEventCreate(&e));
+        const Rep* one = (cnt == 1) ? h_reps : nullptr;
+        BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 0, tw, s, one, 0));
+        float t[lfts::SLAB_VARIANTS];
+        for (int v = 0; v < lfts::SLAB_VARIANTS; v++) {
+            BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 1, tw, s, one, v));  // warm-up
+            BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 2, tw, s, one, v));
+            BHIPCHK(b, hipEventRecord(b->tune_ev[0], s));
+            for (int rep = 0; rep < 4; rep++) {
+                BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 1, tw, s, one, v));
+                BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 2, tw, s, one, v));
+            }
+            BHIPCHK(b, hipEventRecord(b->tune_ev[1], s));
+            BHIPCHK(b, hipEventSynchronize(b->tune_ev[1]));
+            BHIPCHK(b, hipEventElapsedTime(&t[v], b->tune_ev[0], b->tune_ev[1]));
+            if (t[v] < t[best]) best = v;
+        }
+        std::snprintf(line, sizeof line, "edge %d x %d (tuned on %d): variant %d (us per pass: %.2f %.2f %.2f %.2f)\n",
+                      edge, cls, cnt, best, 125.0 * t[0], 125.0 * t[1], 125.0 * t[2], 125.0 * t[3]);
+    }
+    b->tune_report += line;
+    b->tuned[key] = best;
+    *variant = best;
+    return LFTS_OK;
+}
+
 // One propagator evaluation (src/diblock.cc:30-60) of every active replica; Q of
 // replica id lands in d_scal[id][0].
 int calc_concs_async(lfts_batch* b) {
@@ -299,6 +349,13 @@ int calc_concs_async(lfts_batch* b) {
     // more than one mesh size each group's chain runs on its own stream.
     const int ng = (int)b->groups.size();
     const bool fork = ng > 1;
+    // kernel shapes, tuned (on first use of a size class) before the groups run concurrently
+    int variant[3] = {0, 0, 0};
+    for (int g = 0; g < ng; g++) {
+        const auto& G = b->groups[g];
+        if (int rc = slab_variant(b, G.edge, G.cnt, b->d_ract + G.off, b->h_ract + G.off, G.tw, b->st, &variant[g]))
+            return rc;
+    }
     if (fork) {
         BHIPCHK(b, hipEventRecord(b->ev_fork, b->st));
         for (int g = 0; g < ng; g++) BHIPCHK(b, hipStreamWaitEvent(b->gst[g], b->ev_fork, 0));
@@ -308,8 +365,9 @@ int calc_concs_async(lfts_batch* b) {
         hipStream_t s = fork ? b->gst[g] : b->st;
         int Ng = 0;
         for (int k = 0; k < G.cnt; k++) Ng = std::max(Ng, b->ctx[b->act[G.off + k]]->N);
+        const int v = variant[g];
         for (int i = 0; i < Ng; i++) {
-            BHIPCHK(b, lfts::launch_slab(b->d_ract + G.off, G.cnt, G.edge, i, G.tw, s, b->h_ract + G.off));
+            BHIPCHK(b, lfts::launch_slab(b->d_ract + G.off, G.cnt, G.edge, i, G.tw, s, b->h_ract + G.off, v));
             if (i > 0) npasses++;
         }
     }
@@ -825,6 +883,8 @@ int lfts_batch_destroy(lfts_batch* b) {
     for (auto& q : b->ev_pending)
         for (hipEvent_t e : {q.pe0, q.ch0, q.ch1, q.pe1}) b->ev_pool.push_back(e);
     for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->tune_ev)
+        if (e) (void)hipEventDestroy(e);
     for (int g = 0; g < 3; g++) {
         if (b->gst[g]) {
             (void)hipStreamSynchronize(b->gst[g]);
@@ -875,6 +935,10 @@ int lfts_batch_full_step_dev(lfts_batch* b, const double* const* eta_new_dev, co
     const int rc2 = batch_calc_concs(b, lnQ);
     return rc ? rc : rc2;
 }
+
+const char* lfts_batch_tuning(const lfts_batch* b) { return b ? b->tune_report.c_str() : ""; }
+
+const char* lfts_tuning(const lfts_ctx* c) { return (c && c->self) ? c->self->tune_report.c_str() : ""; }
 
 int lfts_batch_profile(lfts_batch* b, int enable) {
     if (!b) return LFTS_EINVAL;

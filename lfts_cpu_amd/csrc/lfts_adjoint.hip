@@ -374,6 +374,10 @@ int run_chain(lfts_ctx* c, int mode, const double* pin) {
         return (mode == lfts::CHAIN_ADJOINT) ? adj_alpha(c, i, f, aux) : factor(c, mode, i, f, aux);
     };
     if (c->fused) {
+        // the kernel shape the context's own plain chain was tuned to (every shape gives
+        // the same bits)
+        const auto tv = c->self->tuned.find(std::make_pair(c->d.m[0], 1));
+        const int variant = (tv != c->self->tuned.end()) ? tv->second : 0;
         for (int i = 0; i < N; i++) {
             SlabIO io{};
             io.tin = (i == 0) ? pin : ((i & 1) ? c->T0 : c->T1);
@@ -390,7 +394,7 @@ int run_chain(lfts_ctx* c, int mode, const double* pin) {
             io.finish = i >= 1;
             io.cont = i < N - 1;
             io.xslab = (i % 2 == 0);
-            HIPCHK(c, lfts::launch_slab_io(c->d.m[0], mode, io, c->ax[0].tw, st));
+            HIPCHK(c, lfts::launch_slab_io(c->d.m[0], mode, io, c->ax[0].tw, st, variant));
         }
     } else {
         double* tmp = c->T0;

@@ -26,7 +26,14 @@
 // steps with conjugate roots. The kernel factor g includes the 1/n of each inverse.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lfts_internal.h"
+
+// Rounding is fixed by the source, not left to the compiler: fused multiply-adds are
+// written out (fma) where they are wanted and nothing else is contracted, so every
+// kernel shape below computes bit-identical results.
+#pragma clang fp contract(off)
 
 namespace lfts {
 
@@ -35,10 +42,10 @@ namespace {
 __device__ __forceinline__ double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 c_mul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double2 c_mulc(double2 a, double2 b) {  // a * conj(b)
-    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+    return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -(a.x * b.y)));
 }
 // multiply by -i (forward) or +i (inverse)
 template <bool INV>
@@ -135,16 +142,50 @@ __device__ __forceinline__ void dft(double2* v) {
     else if constexpr (R == 2) dft2<INV>(v[0], v[1]);
 }
 
+// Twiddles w_n^{jk} (k < R1) and kernel values g(jU + u + R1 f) of the line lane j a
+// thread serves: held in registers (loaded once per pass) or read from the workgroup's
+// LDS copy of the axis tables when used (fewer registers; 48-point lines need that to
+// run two waves per SIMD).
+template <int R1, int R2>
+struct RegTab {
+    double2 tw[R1];
+    double g[R1];
+    __device__ __forceinline__ void load(const double2* __restrict__ twt, const double* __restrict__ gt, int j) {
+        constexpr int U = R1 / R2;
+#pragma unroll
+        for (int k = 0; k < R1; k++) tw[k] = twt[j * k];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int f = 0; f < R2; f++) g[u * R2 + f] = gt[j * U + u + R1 * f];
+    }
+    __device__ __forceinline__ double2 w(int k) const { return tw[k]; }
+    __device__ __forceinline__ double gv(int u, int f) const { return g[u * R2 + f]; }
+};
+
+template <int R1, int R2>
+struct LdsTab {
+    const double2* tw;
+    const double* g;
+    int j;
+    __device__ __forceinline__ void load(const double2* twt, const double* gt, int jj) {
+        tw = twt;
+        g = gt;
+        j = jj;
+    }
+    __device__ __forceinline__ double2 w(int k) const { return tw[j * k]; }
+    __device__ __forceinline__ double gv(int u, int f) const { return g[j * (R1 / R2) + u + R1 * f]; }
+};
+
 // 1D circular convolution of one complex line of length N = R1*R2 held by R2 threads
 // (this thread: index j, elements z[k] = x[j + R2 k]). lx is the line's LDS exchange row
-// (R1*R2 complex). The thread's twiddles tw[k] = exp(-2 pi i jk/N) and kernel values
-// g[u*R2 + f] = g(jU + u + R1 f) are held in registers.
-template <int R1, int R2>
-__device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const double2* tw, const double* g) {
+// (R1*R2 complex); tab supplies the thread's twiddles exp(-2 pi i jk/N) and kernel values.
+template <int R1, int R2, class Tab>
+__device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const Tab& tab) {
     constexpr int U = R1 / R2;
     dft<R1, false>(z);
 #pragma unroll
-    for (int k = 1; k < R1; k++) z[k] = c_mul(z[k], tw[k]);
+    for (int k = 1; k < R1; k++) z[k] = c_mul(z[k], tab.w(k));
     __syncthreads();  // previous readers of the exchange rows are done
 #pragma unroll
     for (int k = 0; k < R1; k++) lx[k * R2 + j] = z[k];
@@ -159,8 +200,9 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
         dft<R2, false>(y + u * R2);
 #pragma unroll
         for (int f = 0; f < R2; f++) {
-            y[u * R2 + f].x *= g[u * R2 + f];
-            y[u * R2 + f].y *= g[u * R2 + f];
+            const double gg = tab.gv(u, f);
+            y[u * R2 + f].x *= gg;
+            y[u * R2 + f].y *= gg;
         }
         dft<R2, true>(y + u * R2);
     }
@@ -173,31 +215,33 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 #pragma unroll
     for (int k = 0; k < R1; k++) z[k] = lx[k * R2 + j];
 #pragma unroll
-    for (int k = 1; k < R1; k++) z[k] = c_mulc(z[k], tw[k]);
+    for (int k = 1; k < R1; k++) z[k] = c_mulc(z[k], tab.w(k));
     dft<R1, true>(z);
 }
 
 }  // namespace
 
-// One slab pass of one replica (plane `plane`, field blockIdx.y), addresses in `io`.
+// One slab pass of one replica (plane `plane`), addresses in `io`. The workgroup serves
+// field blockIdx.y, or with FPW = 2 both fields (threads [0, TPF) field 0, [TPF, 2 TPF)
+// field 1). TWL = 1 keeps the twiddle and kernel tables in LDS instead of registers.
 // MODE selects the finishing rule of pair i+1 from u = (C x)(r):
 //   CHAIN_PLAIN    out = h u                   (src/step.cc:53-64)
 //   CHAIN_TANGENT  out = h u + aux src         (directional derivative of the chain)
 //   CHAIN_ADJOINT  out = u + aux src, stored; the chain continues with h out
-template <int E, int R1, int R2, int MODE>
+// Every variant performs the same arithmetic on every element: results are bit-identical.
+template <int E, int R1, int R2, int MODE, int TWL, int FPW>
 __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const double2* __restrict__ tw) {
     static_assert(R1 * R2 == E && R1 % R2 == 0, "unsupported factorisation");
     constexpr int H = E / 2;           // complex lines per plane
-    constexpr int U = R1 / R2;
+    constexpr int TPF = H * R2;        // threads per field
     constexpr int XP = E + 1;          // exchange row pitch (complex)
     constexpr int RP = E + 1;          // real plane row pitch (doubles)
     constexpr int XBYTES = H * XP * 16;
     constexpr int RBYTES = E * RP * 8;
     constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
+    constexpr int TBYTES = TWL ? E * 16 + 2 * E * 8 : 0;
     constexpr bool EXTRA = MODE != CHAIN_PLAIN;
-    __shared__ __align__(16) unsigned char smem[BYTES];
-    double2* xbuf = reinterpret_cast<double2*>(smem);
-    double* rp = reinterpret_cast<double*>(smem);
+    __shared__ __align__(16) unsigned char smem[FPW * BYTES + TBYTES];
 
     const bool xslab = io.xslab != 0;
     const bool finish = io.finish != 0, cont = io.cont != 0;
@@ -207,8 +251,11 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     const double* ga = xslab ? io.g[1] : io.g[0];
     const double* gb = io.g[2];
 
-    const int t = threadIdx.x;
-    const int f = blockIdx.y;
+    const int fl = (FPW == 2) ? (int)(threadIdx.x / TPF) : 0;
+    const int t = threadIdx.x - fl * TPF;
+    const int f = (FPW == 2) ? fl : (int)blockIdx.y;
+    double2* xbuf = reinterpret_cast<double2*>(smem + fl * BYTES);
+    double* rp = reinterpret_cast<double*>(smem + fl * BYTES);
     const long pbase = (long)plane * plane_stride;  // plane origin within a field
     const long base = (long)f * M + pbase;
     const int c = t % H, j = t / H;    // layout A: a-lines, column pair (c, c + H), a = j + R2 k
@@ -233,23 +280,27 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
             }
         }
     }
-    double2 twA[R1], twB[R1];
-    double gA[R1], gB[R1];
-#pragma unroll
-    for (int k = 0; k < R1; k++) {
-        twA[k] = tw[j * k];
-        twB[k] = tw[jb * k];
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-        for (int ff = 0; ff < R2; ff++) {
-            gA[u * R2 + ff] = ga[j * U + u + R1 * ff];
-            gB[u * R2 + ff] = gb[jb * U + u + R1 * ff];
+    using Tab = typename std::conditional<TWL != 0, LdsTab<R1, R2>, RegTab<R1, R2>>::type;
+    Tab tabA, tabB;
+    if constexpr (TWL != 0) {
+        double2* stw = reinterpret_cast<double2*>(smem + FPW * BYTES);
+        double* sga = reinterpret_cast<double*>(smem + FPW * BYTES + E * 16);
+        double* sgb = sga + E;
+        for (int k = threadIdx.x; k < E; k += blockDim.x) {
+            stw[k] = tw[k];
+            sga[k] = ga[k];
+            sgb[k] = gb[k];
         }
+        __syncthreads();
+        tabA.load(stw, sga, j);
+        tabB.load(stw, sgb, jb);
+    } else {
+        tabA.load(tw, ga, j);
+        tabB.load(tw, gb, jb);
+    }
 
     if (finish) {
-        conv_line<R1, R2>(z, xbuf + c * XP, j, twA, gA);
+        conv_line<R1, R2>(z, xbuf + c * XP, j, tabA);
         double* qout = io.qout;
 #pragma unroll
         for (int k = 0; k < R1; k++) {
@@ -258,11 +309,11 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
                 z[k].x *= hz[k].x;
                 z[k].y *= hz[k].y;
             } else if constexpr (MODE == CHAIN_TANGENT) {
-                z[k].x = z[k].x * hz[k].x + az[k].x * sz[k].x;
-                z[k].y = z[k].y * hz[k].y + az[k].y * sz[k].y;
+                z[k].x = fma(z[k].x, hz[k].x, az[k].x * sz[k].x);
+                z[k].y = fma(z[k].y, hz[k].y, az[k].y * sz[k].y);
             } else {
-                z[k].x += az[k].x * sz[k].x;
-                z[k].y += az[k].y * sz[k].y;
+                z[k].x = fma(az[k].x, sz[k].x, z[k].x);
+                z[k].y = fma(az[k].y, sz[k].y, z[k].y);
             }
             qout[p] = z[k].x;
             qout[p + H] = z[k].y;
@@ -274,7 +325,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     }
     if (!cont) return;
 
-    conv_line<R1, R2>(z, xbuf + c * XP, j, twA, gA);
+    conv_line<R1, R2>(z, xbuf + c * XP, j, tabA);
     // re-pair the real plane: rows (r, r + H) become the complex z-lines
     __syncthreads();
 #pragma unroll
@@ -290,7 +341,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
         const int b = jb + R2 * k;
         w[k] = make_double2(rp[r * RP + b], rp[(r + H) * RP + b]);
     }
-    conv_line<R1, R2>(w, xbuf + r * XP, jb, twB, gB);
+    conv_line<R1, R2>(w, xbuf + r * XP, jb, tabB);
     double* Tout = io.tout;
 #pragma unroll
     for (int k = 0; k < R1; k++) {
@@ -302,56 +353,81 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
 
 // Plain chains of a replica set. Workgroup (slot * E + plane, field): slot picks the
 // replica descriptor.
-template <int E, int R1, int R2>
-__global__ __launch_bounds__((E / 2) * R2) void k_slab(const Rep* __restrict__ reps, const double2* __restrict__ tw,
-                                                     int i) {
+template <int E, int R1, int R2, int TWL, int FPW>
+__global__ __launch_bounds__((E / 2) * R2 * FPW) void k_slab(const Rep* __restrict__ reps,
+                                                           const double2* __restrict__ tw, int i) {
     const int slot = blockIdx.x / E, plane = blockIdx.x - slot * E;
     const Rep& R = reps[slot];  // fields read in place (a private copy would live in scratch)
     if (i > R.N - 1) return;    // this replica's chain is complete
     const SlabIO io = plain_io(R, i);
-    slab_pass<E, R1, R2, CHAIN_PLAIN>(io, plane, tw);
+    slab_pass<E, R1, R2, CHAIN_PLAIN, TWL, FPW>(io, plane, tw);
 }
 
 // One replica, its pass addresses in the kernel arguments (no dependent descriptor load
 // before the first field loads).
-template <int E, int R1, int R2, int MODE>
-__global__ __launch_bounds__((E / 2) * R2) void k_slab_io(const SlabIO io, const double2* __restrict__ tw) {
-    slab_pass<E, R1, R2, MODE>(io, blockIdx.x, tw);
+template <int E, int R1, int R2, int MODE, int TWL, int FPW>
+__global__ __launch_bounds__((E / 2) * R2 * FPW) void k_slab_io(const SlabIO io, const double2* __restrict__ tw) {
+    slab_pass<E, R1, R2, MODE, TWL, FPW>(io, blockIdx.x, tw);
 }
 
 bool slab_supported(int n0, int n1, int n2) {
     return n0 == n1 && n1 == n2 && (n0 == 32 || n0 == 48 || n0 == 64);
 }
 
-hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st, const Rep* one) {
-    if (one && nact == 1) {
-        if (i > one->N - 1) return hipSuccess;
-        return launch_slab_io(n, CHAIN_PLAIN, plain_io(*one, i), tw, st);
+namespace {
+
+template <int E, int R1, int R2, int TWL, int FPW>
+void launch_set(const Rep* reps, int nact, int i, const double2* tw, hipStream_t st) {
+    const dim3 grid(nact * E, 2 / FPW), block((E / 2) * R2 * FPW);
+    hipLaunchKernelGGL((k_slab<E, R1, R2, TWL, FPW>), grid, block, 0, st, reps, tw, i);
+}
+
+template <int E, int R1, int R2, int TWL, int FPW>
+void launch_one(int mode, const SlabIO& io, const double2* tw, hipStream_t st) {
+    const dim3 grid(E, 2 / FPW), block((E / 2) * R2 * FPW);
+    if (mode == CHAIN_PLAIN)
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN, TWL, FPW>), grid, block, 0, st, io, tw);
+    else if (mode == CHAIN_TANGENT)
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_TANGENT, TWL, FPW>), grid, block, 0, st, io, tw);
+    else
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_ADJOINT, TWL, FPW>), grid, block, 0, st, io, tw);
+}
+
+// variant v: bit 0 = tables in LDS, bit 1 = both fields in one workgroup
+template <int E, int R1, int R2>
+void dispatch(int v, const Rep* reps, int nact, int i, int mode, const SlabIO* io, const double2* tw,
+              hipStream_t st) {
+    switch (v) {
+        case 1: io ? launch_one<E, R1, R2, 1, 1>(mode, *io, tw, st) : launch_set<E, R1, R2, 1, 1>(reps, nact, i, tw, st); break;
+        case 2: io ? launch_one<E, R1, R2, 0, 2>(mode, *io, tw, st) : launch_set<E, R1, R2, 0, 2>(reps, nact, i, tw, st); break;
+        case 3: io ? launch_one<E, R1, R2, 1, 2>(mode, *io, tw, st) : launch_set<E, R1, R2, 1, 2>(reps, nact, i, tw, st); break;
+        default: io ? launch_one<E, R1, R2, 0, 1>(mode, *io, tw, st) : launch_set<E, R1, R2, 0, 1>(reps, nact, i, tw, st); break;
     }
-    const dim3 grid(nact * n, 2);
-#define SLAB(E, R1, R2) hipLaunchKernelGGL((k_slab<E, R1, R2>), grid, dim3((E / 2) * R2), 0, st, reps, tw, i)
-    if (n == 64) SLAB(64, 8, 8);
-    else if (n == 48) SLAB(48, 12, 4);
-    else if (n == 32) SLAB(32, 8, 4);
+}
+
+hipError_t dispatch_edge(int n, int v, const Rep* reps, int nact, int i, int mode, const SlabIO* io,
+                         const double2* tw, hipStream_t st) {
+    if (n == 64) dispatch<64, 8, 8>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 48) dispatch<48, 12, 4>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 32) dispatch<32, 8, 4>(v, reps, nact, i, mode, io, tw, st);
     else return hipErrorInvalidValue;
-#undef SLAB
     return hipGetLastError();
 }
 
-hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st) {
-#define SLABIO(E, R1, R2)                                                                                      \
-    do {                                                                                                       \
-        const dim3 grid(E, 2), block((E / 2) * R2);                                                            \
-        if (mode == CHAIN_PLAIN) hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN>), grid, block, 0, st, io, tw);     \
-        else if (mode == CHAIN_TANGENT) hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_TANGENT>), grid, block, 0, st, io, tw); \
-        else hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_ADJOINT>), grid, block, 0, st, io, tw);                 \
-    } while (0)
-    if (n == 64) SLABIO(64, 8, 8);
-    else if (n == 48) SLABIO(48, 12, 4);
-    else if (n == 32) SLABIO(32, 8, 4);
-    else return hipErrorInvalidValue;
-#undef SLABIO
-    return hipGetLastError();
+}  // namespace
+
+hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st, const Rep* one,
+                       int variant) {
+    if (one && nact == 1) {
+        if (i > one->N - 1) return hipSuccess;
+        const SlabIO io = plain_io(*one, i);
+        return dispatch_edge(n, variant, nullptr, 1, i, CHAIN_PLAIN, &io, tw, st);
+    }
+    return dispatch_edge(n, variant, reps, nact, i, CHAIN_PLAIN, nullptr, tw, st);
+}
+
+hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st, int variant) {
+    return dispatch_edge(n, variant, nullptr, 1, 0, mode, &io, tw, st);
 }
 
 }  // namespace lfts

@@ -131,10 +131,15 @@ __host__ __device__ inline SlabIO plain_io(const Rep& R, int i) {
 // replicas with edge n; when nact == 1 and `one` (a host copy of that descriptor) is
 // given, the descriptor travels in the kernel arguments instead.
 bool slab_supported(int n0, int n1, int n2);
+// `variant` selects one of SLAB_VARIANTS kernel shapes (bit 0: twiddle/kernel tables in
+// LDS instead of registers; bit 1: both fields of a plane in one workgroup). All variants
+// give bit-identical results; the fastest one per mesh size and set size is picked by
+// timing (the autotuner of the replica sets).
+constexpr int SLAB_VARIANTS = 4;
 hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st,
-                       const Rep* one = nullptr);
-// One pass of one replica's chain of kind `mode` (CHAIN_*), grid (n planes, 2 fields).
-hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st);
+                       const Rep* one = nullptr, int variant = 0);
+// One pass of one replica's chain of kind `mode` (CHAIN_*), over the n planes.
+hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st, int variant = 0);
 
 // ---- batched element-wise and reduction kernels ----------------------------------------
 // reps: descriptors of the nact active replicas; grid (nb_max, nact).

@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "lfts_gpu.h"
@@ -81,6 +83,10 @@ struct lfts_batch {
     hipStream_t gst[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
+    // slab kernel variant per (mesh edge, members in the launch), picked by timing
+    std::map<std::pair<int, int>, int> tuned;
+    std::string tune_report;
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};
     // live event timing
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -114,6 +120,8 @@ int line_contour_step(lfts_ctx* c, const double* in, double* out, const double* 
                       hipStream_t st);
 int set_active(lfts_batch* b, const std::vector<int>& ids);
 int calc_concs_async(lfts_batch* b);
+int slab_variant(lfts_batch* b, int edge, int cnt, const lfts::Rep* d_reps, const lfts::Rep* h_reps,
+                 const double2* tw, hipStream_t s, int* variant);
 int read_scal(lfts_batch* b);
 int sync_batch(lfts_batch* b);
 std::vector<int> all_ids(const lfts_batch* b);
