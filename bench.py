@@ -31,6 +31,7 @@ if ROOT not in sys.path:
 
 METRIC = "Langevin steps/s at 64³,N=90 FP64 (plain vs batched); % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
+NK_STEPS = 10  # Langevin steps timed with the Newton-Krylov solver in the derivative leg
 
 
 def log(*a):
@@ -183,12 +184,14 @@ def batched_leg(steps, warmup, L_mod, torch, quick=False):
     }
 
 
-def adjoint_leg(ctx, eta_dev, XbN, dt, M, torch, tols=(1e-4, 1e-12)):
+def adjoint_leg(ctx, eta_dev, eta_more, XbN, dt, M, torch, tols=(1e-4, 1e-12)):
     """Newton-Krylov against Anderson on identical W-: from the state after the timed
     steps, one more W- update (as at the start of every Langevin step), then both
     solvers from the same fields to the same residual, at the production tolerance and
     at a tight one (where the two w+ must agree). Also times the tangent and adjoint
-    products (each includes its forward evaluation) and the box derivative."""
+    products (each includes its forward evaluation) and the box derivative, and whole
+    Langevin steps with the Newton-Krylov solve in place of Anderson (a different solver:
+    an extension, not the reference's algorithm)."""
     ctx.langevin_step_dev(eta_dev, XbN, dt)
     w_start = ctx.get_fields(2)
     ctx.saddle_nk(tol=1e-2)  # one-time set-up (workspace, preconditioner table) outside the timing
@@ -218,6 +221,16 @@ def adjoint_leg(ctx, eta_dev, XbN, dt, M, torch, tols=(1e-4, 1e-12)):
             "nk_ms": 1e3 * t_nk,
             "max_abs_dwplus": float(np.abs(w_nk[M:] - w_and[M:]).max()),
         })
+    # whole Langevin steps with the Newton-Krylov solve (tol 1e-4) in place of Anderson,
+    # from the same start, with the next noise vectors of the reference stream
+    ctx.set_fields(w_start)
+    nk_steps = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e in eta_more:
+        _, inf = ctx.full_step_nk_dev(e.data_ptr(), XbN, dt, tol=1e-4)
+        nk_steps.append(inf["prop_evals"] + inf["krylov"])
+    t_steps = time.perf_counter() - t0
     dev = torch.empty(6 * M, dtype=torch.float64, device="cuda")
     g = torch.Generator(device="cpu").manual_seed(5)
     dev[:3 * M].copy_(torch.randn(3 * M, generator=g, dtype=torch.float64))
@@ -238,6 +251,8 @@ def adjoint_leg(ctx, eta_dev, XbN, dt, M, torch, tols=(1e-4, 1e-12)):
         "workload": "64^3 N=90 state after the timed steps plus one W- update; Anderson (10 histories) and "
                     "Newton-Krylov (GMRES(20), homogeneous-melt preconditioner) from the same fields",
         "solves": rows,
+        "nk_langevin_steps_per_s": len(eta_more) / t_steps if eta_more else None,
+        "nk_chain_evals_per_step": float(np.mean(nk_steps)) if nk_steps else None,
         "jvp_with_forward_ms": 1e3 * t_jvp,
         "vjp_with_forward_ms": 1e3 * t_vjp,
         "box_derivative_ms": 1e3 * t_box,
@@ -287,8 +302,9 @@ def main():
     log(f"initial mix: k={k0} err={err0:.3e} ({time.perf_counter() - t0:.2f} s)")
 
     nsteps = args.warmup + args.steps
-    eta = torch.empty((nsteps + 1, M), dtype=torch.float64, device="cuda")
-    for s in range(nsteps + 1):
+    n_extra = 0 if args.no_adjoint else 1 + NK_STEPS
+    eta = torch.empty((nsteps + n_extra, M), dtype=torch.float64, device="cuda")
+    for s in range(nsteps + n_extra):
         eta[s].copy_(torch.from_numpy(rng.normals(M)))
     torch.cuda.synchronize()
 
@@ -328,7 +344,8 @@ def main():
 
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_leg(ctx, eta[nsteps].data_ptr(), XbN, dt, M, torch)
+        adjoint = adjoint_leg(ctx, eta[nsteps].data_ptr(), [eta[nsteps + 1 + k] for k in range(NK_STEPS)], XbN, dt,
+                              M, torch)
         log(f"adjoint leg: {adjoint}")
 
     batched = None

@@ -195,6 +195,11 @@ typedef struct lfts_nk_info {
     double err;        /* final residual */
 } lfts_nk_info;
 int lfts_saddle_nk(lfts_ctx* ctx, const lfts_nk_opts* opts, lfts_nk_info* info);
+/* One Langevin step with lfts_saddle_nk in place of anderson::mix: step_wm with device
+ * noise, the solve, and lnQ of the final fields (src/lfts_simulation.cc:72-76 with the
+ * other solver; the trajectory differs from the reference's within the tolerance). */
+int lfts_langevin_full_step_nk_dev(lfts_ctx* ctx, const double* eta_new_dev, double XbN, double dt,
+                                   const lfts_nk_opts* opts, double* lnQ, lfts_nk_info* info);
 
 /* Host random stream of the reference: std::mt19937_64 seeded by the caller (the
  * reference seeds 123456789, src/lfts_simulation.cc:77) feeding one persistent

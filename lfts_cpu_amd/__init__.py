@@ -94,6 +94,8 @@ SIGNATURES = {
     "lfts_vjp_dev": (ctypes.c_int, [_vp, _vp, _vp]),
     "lfts_box_derivative": (ctypes.c_int, [_vp, _dp, _dp]),
     "lfts_saddle_nk": (ctypes.c_int, [_vp, ctypes.POINTER(NkOpts), ctypes.POINTER(NkInfo)]),
+    "lfts_langevin_full_step_nk_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double,
+                                                      ctypes.POINTER(NkOpts), _dp, ctypes.POINTER(NkInfo)]),
     "lfts_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
     "lfts_batch_destroy": (ctypes.c_int, [_vp]),
     "lfts_batch_last_error": (ctypes.c_char_p, [_vp]),
@@ -300,6 +302,16 @@ class Context:
         info = NkInfo()
         self._chk(lib().lfts_saddle_nk(self.h, ctypes.byref(o), ctypes.byref(info)))
         return {"newton": info.newton, "krylov": info.krylov, "prop_evals": info.prop_evals, "err": info.err}
+
+    def full_step_nk_dev(self, eta_dev_ptr: int, XbN: float, dt: float, tol: float = 1e-4, maxNewton: int = 30,
+                         restart: int = 20, maxInner: int = 200, eta: float = 0.1, precondition: bool = True):
+        o = NkOpts(int(maxNewton), float(tol), int(restart), int(maxInner), float(eta), int(bool(precondition)))
+        info = NkInfo()
+        lnQ = ctypes.c_double()
+        self._chk(lib().lfts_langevin_full_step_nk_dev(self.h, ctypes.c_void_p(eta_dev_ptr), float(XbN), float(dt),
+                                                       ctypes.byref(o), ctypes.byref(lnQ), ctypes.byref(info)))
+        return lnQ.value, {"newton": info.newton, "krylov": info.krylov, "prop_evals": info.prop_evals,
+                           "err": info.err}
 
     def profile(self, enable: bool = True):
         self._chk(lib().lfts_profile(self.h, int(enable)))

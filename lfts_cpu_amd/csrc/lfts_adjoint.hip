@@ -786,6 +786,21 @@ int lfts_saddle_nk(lfts_ctx* c, const lfts_nk_opts* opts, lfts_nk_info* info) {
     return saddle_nk(c, *opts, info);
 }
 
+// A Langevin step with the Newton-Krylov solve in place of anderson::mix: step_wm, the
+// solve, and lnQ of the final fields, which the solve's last propagator evaluation
+// already computed (the separate calc_concs of src/lfts_simulation.cc:76 is not needed).
+int lfts_langevin_full_step_nk_dev(lfts_ctx* c, const double* eta_new_dev, double XbN, double dt,
+                                   const lfts_nk_opts* opts, double* lnQ, lfts_nk_info* info) {
+    if (!c || !eta_new_dev || !opts) return fail(c, LFTS_EINVAL, "full_step_nk_dev: null argument");
+    if (int rc = lfts_langevin_step_dev(c, eta_new_dev, XbN, dt)) return rc;
+    if (int rc = lfts_saddle_nk(c, opts, info)) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->hred + R_NRM + 1, Qdev(c), sizeof(double), hipMemcpyDeviceToHost, c->self->st));
+    HIPCHK(c, hipStreamSynchronize(c->self->st));
+    const double Q = c->hred[R_NRM + 1];
+    if (lnQ) *lnQ = std::log(Q);
+    return (std::isfinite(Q) && Q > 0) ? LFTS_OK : fail(c, LFTS_ENUM, "partition function Q is not finite and positive");
+}
+
 int lfts_jvp_dev(lfts_ctx* c, const double* dWm, const double* dwp, double* dphi) {
     if (!c || !dphi) return fail(c, LFTS_EINVAL, "jvp: null argument");
     if (int rc = jvp_dev(c, dWm, dwp, dphi, true)) return rc;

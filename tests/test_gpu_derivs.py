@@ -199,3 +199,25 @@ def test_autograd_function_gradcheck(gpu):
     f = lambda a, b: PhiPlus.apply(a, b, ctx)  # noqa: E731
     assert torch.autograd.gradcheck(f, (wp, wm), eps=1e-6, atol=1e-7, rtol=1e-6, check_forward_ad=True,
                                     check_backward_ad=True, check_undefined_grad=False, check_batched_grad=False)
+
+
+def test_langevin_step_with_newton_krylov_is_the_composition(gpu):
+    import torch
+
+    m, Lb, N, NA = (32, 32, 32), (4.0, 4.0, 4.0), 24, 12
+    M = int(np.prod(m))
+    w0 = _lamellar(m, 8.0, 9)
+    rng = np.random.default_rng(10)
+    eta_prev, eta_new = rng.normal(0, 0.5, M), rng.normal(0, 0.5, M)
+    e = torch.from_numpy(eta_new).cuda()
+    a, b = L.Context(N, NA, m, Lb), L.Context(N, NA, m, Lb)
+    for c in (a, b):
+        c.set_fields(w0)
+        c.langevin_init(eta_prev)
+    lnq_a, info_a = a.full_step_nk_dev(e.data_ptr(), 17.0, 0.5, tol=1e-6)
+    b.langevin_step_dev(e.data_ptr(), 17.0, 0.5)
+    info_b = b.saddle_nk(tol=1e-6)
+    lnq_b = b.calc_concs()
+    assert info_a == info_b and info_a["err"] <= 1e-6
+    assert lnq_a == lnq_b
+    assert np.array_equal(a.get_fields(4), b.get_fields(4))
