@@ -102,6 +102,18 @@ int lfts_sk_save(const char* fileName, const int* m, const double* L, const doub
  * H = -lnQ + (sum W-^2 / XbN - sum w+) / M. */
 int lfts_hamiltonian(lfts_ctx* ctx, double XbN, double* H);
 
+/* Storage of the propagator pairs (no reference counterpart: the reference stores all
+ * N pairs, src/diblock.cc:9-25). stride 0 stores every pair; stride S >= 2 keeps about
+ * N/S of them, at the starts of segments chosen symmetric under bead j -> N+1-j, and
+ * recomputes each segment (with its mirror) when the bead sums need it: calc_concs then
+ * costs two chains instead of one, tangent and adjoint products three more. lfts_create
+ * picks stride ceil(sqrt(N)) by itself when the pairs would not fit comfortably in free
+ * HBM. With a stride, lfts_get_pair (i > 1) and lfts_box_derivative are unavailable and
+ * the context cannot join a replica set. lfts_pair_storage reports the stride and the
+ * number of pairs kept. */
+int lfts_set_pair_storage(lfts_ctx* ctx, int stride);
+int lfts_pair_storage(lfts_ctx* ctx, int* stride, int* nkept);
+
 /* Testing access: stored propagator pair i (1..N) as the reference lays it out,
  * [q_i | q+_{N+1-i}] (src/diblock.cc:19-24), 2*M host doubles. */
 int lfts_get_pair(lfts_ctx* ctx, int i, double* out);

@@ -96,6 +96,8 @@ SIGNATURES = {
     "lfts_saddle_nk": (ctypes.c_int, [_vp, ctypes.POINTER(NkOpts), ctypes.POINTER(NkInfo)]),
     "lfts_langevin_full_step_nk_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double,
                                                       ctypes.POINTER(NkOpts), _dp, ctypes.POINTER(NkInfo)]),
+    "lfts_set_pair_storage": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lfts_pair_storage": (ctypes.c_int, [_vp, _ip, _ip]),
     "lfts_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
     "lfts_batch_destroy": (ctypes.c_int, [_vp]),
     "lfts_batch_last_error": (ctypes.c_char_p, [_vp]),
@@ -325,6 +327,15 @@ class Context:
 
     def tuning(self) -> str:
         return lib().lfts_tuning(self.h).decode()
+
+    def set_pair_storage(self, stride: int):
+        self._chk(lib().lfts_set_pair_storage(self.h, int(stride)))
+
+    def pair_storage(self):
+        """(stride, pairs kept)"""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._chk(lib().lfts_pair_storage(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def counters(self):
         a = ctypes.c_longlong()

@@ -336,6 +336,7 @@ int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep*
 int calc_concs_async(lfts_batch* b) {
     const int nact = (int)b->act.size();
     if (nact == 0) return LFTS_OK;
+    if (b->ctx.size() == 1 && b->ctx[0]->ck_stride) return ckpt_calc_concs(b->ctx[0]);
     lfts_batch::EvQuad ev{};
     if (b->prof) {
         for (hipEvent_t* e : {&ev.pe0, &ev.ch0, &ev.ch1, &ev.pe1})
@@ -595,7 +596,16 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     }
     CREATE_HIP(hipMalloc(&c->w, sizeof(double) * 4 * M));
     CREATE_HIP(hipMalloc(&c->h, sizeof(double) * 2 * M));
-    CREATE_HIP(hipMalloc(&c->q, sizeof(double) * 2 * M * (size_t)(c->N - 1)));
+    // the stored pairs: all of them when they fit comfortably in free HBM, else about
+    // sqrt(N) of them at segment starts (checkpointed recomputation)
+    int auto_stride = 0;
+    {
+        size_t free_b = 0, total_b = 0;
+        const double need = 2.0 * sizeof(double) * 2 * M * (double)(c->N - 1);  // pairs + a derivative chain
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && need > 0.5 * (double)free_b)
+            auto_stride = std::max(2, (int)std::ceil(std::sqrt((double)c->N)));
+    }
+    if (!auto_stride) CREATE_HIP(hipMalloc(&c->q, sizeof(double) * 2 * M * (size_t)(c->N - 1)));
     CREATE_HIP(hipMalloc(&c->T0, sizeof(double) * 2 * M));
     if (c->fused) CREATE_HIP(hipMalloc(&c->T1, sizeof(double) * 2 * M));
     CREATE_HIP(hipMalloc(&c->Dh, sizeof(double) * M * (size_t)(c->nhMax + 1)));
@@ -611,6 +621,7 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     c->V.assign(c->nhMax, 0.0);
     c->C.assign(c->nhMax, 0.0);
     CREATE_CHK(batch_create(&c, 1, &c->self));
+    if (auto_stride) CREATE_CHK(set_pair_storage(c, auto_stride));
 #undef CREATE_CHK
 #undef CREATE_HIP
     *out = c;
@@ -620,6 +631,7 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
 int lfts_destroy(lfts_ctx* c) {
     if (!c) return LFTS_EINVAL;
     if (c->self) lfts_batch_destroy(c->self);
+    release_pair_storage(c);
     for (auto& p : c->ax) {
         (void)hipFree(p.tw);
         (void)hipFree(p.gperm);
@@ -825,6 +837,7 @@ int lfts_hamiltonian(lfts_ctx* c, double XbN, double* H) {
 
 int lfts_get_pair(lfts_ctx* c, int i, double* out) {
     if (!c || !out || i < 1 || i > c->N) return fail(c, LFTS_EINVAL, "get_pair: index out of range");
+    if (c->ck_stride && i > 1) return fail(c, LFTS_EINVAL, "get_pair: pairs are not stored (checkpointed storage)");
     HIPCHK(c, hipMemcpyAsync(out, pair_ptr(c, i), sizeof(double) * 2 * c->M, hipMemcpyDeviceToHost, c->self->st));
     HIPCHK(c, hipStreamSynchronize(c->self->st));
     return LFTS_OK;
@@ -871,6 +884,7 @@ int lfts_batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
         if (!ctxs[i]) return LFTS_EINVAL;
         for (int j = 0; j < i; j++)
             if (ctxs[j] == ctxs[i]) return LFTS_EINVAL;
+        if (ctxs[i]->ck_stride) return LFTS_EINVAL;  // members keep every pair
         // work queued on the contexts' own streams finishes first
         if (hipStreamSynchronize(ctxs[i]->self->st) != hipSuccess) return LFTS_EHIP;
     }

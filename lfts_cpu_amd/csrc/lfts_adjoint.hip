@@ -310,7 +310,7 @@ double* RES(lfts_ctx* c) { return c->dred + RED_RES; }
 
 int ensure_deriv(lfts_ctx* c) {
     const long M = c->M;
-    if (!c->dq) HIPCHK(c, hipMalloc(&c->dq, sizeof(double) * 2 * M * (size_t)(c->N - 1)));
+    if (!c->dq && !c->ck_stride) HIPCHK(c, hipMalloc(&c->dq, sizeof(double) * 2 * M * (size_t)(c->N - 1)));
     if (!c->dbuf) HIPCHK(c, hipMalloc(&c->dbuf, sizeof(double) * 10 * M));
     if (!c->dred) HIPCHK(c, hipMalloc(&c->dred, sizeof(double) * (RED_RES + NRES)));
     if (!c->hred) HIPCHK(c, hipHostMalloc(&c->hred, sizeof(double) * NRES, hipHostMallocDefault));
@@ -329,6 +329,21 @@ int multi_dot(lfts_ctx* c, const double* x, const double* Y, long ystride, int n
     HIPCHK(c, hipGetLastError());
     return LFTS_OK;
 }
+
+}  // namespace
+
+int dev_sum(lfts_ctx* c, const double* x, const double* y, long n, double scale, double* out) {
+    if (int rc = ensure_deriv(c)) return rc;
+    hipStream_t st = c->self->st;
+    const int nb = grid_for(n);
+    hipLaunchKernelGGL(lfts::k_multi_dot, dim3(nb, 1), dim3(256), 0, st, x, y, 0L, 1, n, c->dred);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(lfts::k_final, dim3(1), dim3(256), 0, st, (const double*)c->dred, nb, scale, out);
+    HIPCHK(c, hipGetLastError());
+    return LFTS_OK;
+}
+
+namespace {
 
 // The forward evaluation at the current fields: pairs, Q (scal slot 0) and phi.
 int forward(lfts_ctx* c) {
@@ -358,74 +373,103 @@ const double* adj_alpha(const lfts_ctx* c, int i, int f, const double* base) {
     return (bead <= c->NA) ? base : base + c->M;
 }
 
-// Run a tangent or adjoint chain from `pin` (pair 1 for the tangent, the multiplied
-// pair 1 for the adjoint), storing pairs 2..N in c->dq.
-int run_chain(lfts_ctx* c, int mode, const double* pin) {
+}  // namespace
+
+// Passes i0 .. i1-1 of every chain in `chains`, interleaved pass by pass (a derived chain
+// reads the forward pair its pass needs after the forward chain's pass wrote it).
+int run_lockstep(lfts_ctx* c, int i0, int i1, ChainSpec* const* chains, int n) {
+    if (i1 <= i0) return LFTS_OK;
     hipStream_t st = c->self->st;
     const long M = c->M;
-    const int N = c->N;
-    const double* aux = AUX(c);
-    auto src = [&](int i, int f) -> const double* {
-        const double* P = pair_ptr(c, i + 1);
-        if (mode == lfts::CHAIN_ADJOINT) return f ? P : P + M;
+    auto src = [&](const ChainSpec& ch, int i, int f) -> const double* {
+        const double* P = ch.fwd(i + 1);
+        if (ch.mode == lfts::CHAIN_ADJOINT) return f ? P : P + M;
         return f ? P + M : P;
     };
-    auto auxp = [&](int i, int f) -> const double* {
-        return (mode == lfts::CHAIN_ADJOINT) ? adj_alpha(c, i, f, aux) : factor(c, mode, i, f, aux);
+    auto auxp = [&](const ChainSpec& ch, int i, int f) -> const double* {
+        return (ch.mode == lfts::CHAIN_ADJOINT) ? adj_alpha(c, i, f, ch.aux) : factor(c, ch.mode, i, f, ch.aux);
     };
     if (c->fused) {
         // the kernel shape the context's own plain chain was tuned to (every shape gives
         // the same bits)
         const auto tv = c->self->tuned.find(std::make_pair(c->d.m[0], 1));
         const int variant = (tv != c->self->tuned.end()) ? tv->second : 0;
-        for (int i = 0; i < N; i++) {
-            SlabIO io{};
-            io.tin = (i == 0) ? pin : ((i & 1) ? c->T0 : c->T1);
-            io.tout = (i & 1) ? c->T1 : c->T0;
-            io.qout = (i >= 1) ? c->dq + 2L * (i - 1) * M : nullptr;
-            if (i >= 1)
-                for (int f = 0; f < 2; f++) {
-                    io.h[f] = factor(c, mode, i, f, c->h);
-                    io.aux[f] = auxp(i, f);
-                    io.src[f] = src(i, f);
-                }
-            for (int d = 0; d < 3; d++) io.g[d] = c->ax[d].gnat;
-            io.M = M;
-            io.finish = i >= 1;
-            io.cont = i < N - 1;
-            io.xslab = (i % 2 == 0);
-            HIPCHK(c, lfts::launch_slab_io(c->d.m[0], mode, io, c->ax[0].tw, st, variant));
-        }
-    } else {
-        double* tmp = c->T0;
-        for (int i = 1; i < N; i++) {
-            const double* in;
-            if (mode == lfts::CHAIN_ADJOINT) in = D2(c);
-            else in = (i == 1) ? pin : c->dq + 2L * (i - 2) * M;
-            double* out = c->dq + 2L * (i - 1) * M;
-            LineArgs z = line_args(c, 2, lfts::LINE_CONV);
-            z.inA = in; z.inB = in + M; z.outA = tmp; z.outB = tmp + M;
-            HIPCHK(c, lfts::launch_line(z, st));
-            LineArgs y = line_args(c, 1, lfts::LINE_CONV);
-            y.inA = tmp; y.inB = tmp + M; y.outA = tmp; y.outB = tmp + M;
-            HIPCHK(c, lfts::launch_line(y, st));
-            LineArgs x = line_args(c, 0, lfts::LINE_CONV);
-            x.inA = tmp; x.inB = tmp + M; x.outA = out; x.outB = out + M;
-            x.hA = factor(c, mode, i, 0, c->h);
-            x.hB = factor(c, mode, i, 1, c->h);
-            x.chain = mode;
-            x.auxA = auxp(i, 0);
-            x.auxB = auxp(i, 1);
-            x.srcA = src(i, 0);
-            x.srcB = src(i, 1);
-            if (mode == lfts::CHAIN_ADJOINT) {
-                x.out2A = D2(c);
-                x.out2B = D2(c) + M;
+        for (int p = i0 - 1; p < i1; p++)
+            for (int k = 0; k < n; k++) {
+                const ChainSpec& ch = *chains[k];
+                SlabIO io{};
+                io.tin = (p == i0 - 1) ? ch.start : ((p & 1) ? ch.T0 : ch.T1);
+                io.tout = (p & 1) ? ch.T1 : ch.T0;
+                const bool fin = p >= i0;
+                io.qout = fin ? ch.out(p + 1) : nullptr;
+                if (fin)
+                    for (int f = 0; f < 2; f++) {
+                        io.h[f] = factor(c, ch.mode, p, f, c->h);
+                        if (ch.mode != lfts::CHAIN_PLAIN) {
+                            io.aux[f] = auxp(ch, p, f);
+                            io.src[f] = src(ch, p, f);
+                        }
+                    }
+                for (int d = 0; d < 3; d++) io.g[d] = c->ax[d].gnat;
+                io.M = M;
+                io.finish = fin;
+                io.cont = p < i1 - 1;
+                io.xslab = (p % 2 == 0);
+                HIPCHK(c, lfts::launch_slab_io(c->d.m[0], ch.mode, io, c->ax[0].tw, st, variant));
             }
-            HIPCHK(c, lfts::launch_line(x, st));
-        }
+    } else {
+        for (int p = i0; p < i1; p++)
+            for (int k = 0; k < n; k++) {
+                const ChainSpec& ch = *chains[k];
+                const double* in;
+                if (p == i0) in = ch.start;
+                else in = (ch.mode == lfts::CHAIN_ADJOINT) ? ch.post : ch.out(p);
+                double* out = ch.out(p + 1);
+                double* tmp = ch.T0;
+                LineArgs z = line_args(c, 2, lfts::LINE_CONV);
+                z.inA = in; z.inB = in + M; z.outA = tmp; z.outB = tmp + M;
+                HIPCHK(c, lfts::launch_line(z, st));
+                LineArgs y = line_args(c, 1, lfts::LINE_CONV);
+                y.inA = tmp; y.inB = tmp + M; y.outA = tmp; y.outB = tmp + M;
+                HIPCHK(c, lfts::launch_line(y, st));
+                LineArgs x = line_args(c, 0, lfts::LINE_CONV);
+                x.inA = tmp; x.inB = tmp + M; x.outA = out; x.outB = out + M;
+                x.hA = factor(c, ch.mode, p, 0, c->h);
+                x.hB = factor(c, ch.mode, p, 1, c->h);
+                x.chain = ch.mode;
+                if (ch.mode != lfts::CHAIN_PLAIN) {
+                    x.auxA = auxp(ch, p, 0);
+                    x.auxB = auxp(ch, p, 1);
+                    x.srcA = src(ch, p, 0);
+                    x.srcB = src(ch, p, 1);
+                }
+                if (ch.mode == lfts::CHAIN_ADJOINT) {
+                    x.out2A = ch.post;
+                    x.out2B = ch.post + M;
+                }
+                HIPCHK(c, lfts::launch_line(x, st));
+            }
     }
     return LFTS_OK;
+}
+
+namespace {
+
+// A tangent or adjoint chain over all beads from `pin` (pair 1 for the tangent, the
+// multiplied pair 1 for the adjoint), storing pairs 2..N in c->dq.
+int run_chain(lfts_ctx* c, int mode, const double* pin) {
+    const long M = c->M;
+    ChainSpec ch;
+    ch.mode = mode;
+    ch.start = pin;
+    ch.T0 = c->T0;
+    ch.T1 = c->T1;
+    ch.post = D2(c);
+    ch.aux = AUX(c);
+    ch.out = [c, M](int j) { return c->dq + 2L * (j - 2) * M; };
+    ch.fwd = [c](int j) -> const double* { return pair_ptr(c, j); };
+    ChainSpec* cs[1] = {&ch};
+    return run_lockstep(c, 1, c->N, cs, 1);
 }
 
 }  // namespace
@@ -442,6 +486,13 @@ int jvp_dev(lfts_ctx* c, const double* du, const double* dv, double* dphi, bool 
     hipLaunchKernelGGL(lfts::k_tan_prep, dim3(nb), dim3(256), 0, st, M, c->N, (const double*)c->h, du, dv, AUX(c),
                        D1(c));
     HIPCHK(c, hipGetLastError());
+    if (c->ck_stride) {
+        if (int rc = ckpt_derived(c, lfts::CHAIN_TANGENT, D1(c), D1(c))) return rc;
+        if (int rc = multi_dot(c, ckpt_pair_N(c, true), nullptr, 0, 1, M, 1.0 / (double)M, R_DQ)) return rc;
+        if (int rc = ckpt_tan_finish(c, AUX(c), RES(c) + R_DQ, dphi)) return rc;
+        c->n_tan++;
+        return LFTS_OK;
+    }
     if (int rc = run_chain(c, lfts::CHAIN_TANGENT, D1(c))) return rc;
     const double* dqN = c->dq + 2L * (c->N - 2) * M;
     if (int rc = multi_dot(c, dqN, nullptr, 0, 1, M, 1.0 / (double)M, R_DQ)) return rc;
@@ -491,6 +542,12 @@ int vjp_dev(lfts_ctx* c, const double* ubar, double* wbar, bool fresh) {
     a.wbar = wbar;
     hipLaunchKernelGGL(lfts::k_adj_prep, dim3(nb), dim3(256), 0, st, a);
     HIPCHK(c, hipGetLastError());
+    if (c->ck_stride) {
+        if (int rc = ckpt_derived(c, lfts::CHAIN_ADJOINT, D1(c), D2(c))) return rc;
+        if (int rc = ckpt_adj_finish(c, ubar, wbar)) return rc;
+        c->n_adj++;
+        return LFTS_OK;
+    }
     if (int rc = run_chain(c, lfts::CHAIN_ADJOINT, D2(c))) return rc;
     hipLaunchKernelGGL(lfts::k_adj_final, dim3(nb), dim3(256), 0, st, a);
     HIPCHK(c, hipGetLastError());
@@ -850,6 +907,7 @@ int lfts_box_derivative(lfts_ctx* c, double* dHdL, double* lnQ) {
     if (!c || !dHdL) return fail(c, LFTS_EINVAL, "box_derivative: null argument");
     if (c->NA < 1 || c->NA > c->N - 1)
         return fail(c, LFTS_EINVAL, "box_derivative: needs 1 <= NA <= N-1 (the q+ chain is then the q chain's adjoint)");
+    if (c->ck_stride) return fail(c, LFTS_EINVAL, "box_derivative: needs every propagator pair stored");
     if (int rc = ensure_deriv(c)) return rc;
     hipStream_t st = c->self->st;
     const long M = c->M;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <string>
 #include <utility>
@@ -50,6 +51,16 @@ struct lfts_ctx {
     double* acc = nullptr;    // M, spectral accumulator of the box derivative
     double* dred = nullptr;   // reduction partials (32 x MAX_NB) and 64 results
     double* hred = nullptr;   // pinned host copy of the results
+    // checkpointed pair storage (ck_stride > 0): pairs kept only at segment starts
+    int ck_stride = 0;
+    std::vector<int> ck_b;      // segment starts b_0 = 1 < ... < b_{K-1}, and b_K = N + 1
+    int ck_smax = 0;            // longest segment (pairs)
+    double* ck_f = nullptr;     // forward pairs at b_1..b_{K-1}, then pair N
+    double* ck_d = nullptr;     // derived-chain pairs at the same places, then pair N
+    double* ck_seg = nullptr;   // 4 segment buffers of ck_smax pairs
+    double* ck_ring = nullptr;  // 4 pairs: forward pairs in flight, adjoint carry, scratch
+    double* ck_T = nullptr;     // 4M: slab pipeline of a derived chain run beside the forward one
+    double* ck_acc = nullptr;   // 4M: raw bead sums
     // Newton-Krylov workspace (allocated on first use)
     double* kry = nullptr;    // (kry_m + 1) M Krylov basis, then 5M work vectors
     int kry_m = 0;
@@ -127,6 +138,34 @@ int sync_batch(lfts_batch* b);
 std::vector<int> all_ids(const lfts_batch* b);
 void lu_solve(int n, const double* A_in, const double* b, double* x);
 bool factorize(int n, std::vector<int>& r);
+
+// One propagator-type chain for run_lockstep: passes i0 .. i1-1 produce pairs
+// i0+1 .. i1 from pair i0 (`start`; for the adjoint, multiplied by the factors of the
+// next pass). A derived chain (tangent, adjoint) reads the forward pair of each pass
+// through `fwd`, so a forward chain run in the same call must come first.
+struct ChainSpec {
+    int mode = lfts::CHAIN_PLAIN;
+    const double* start = nullptr;
+    double* T0 = nullptr;    // slab pipeline pairs; T0 is the line path's scratch
+    double* T1 = nullptr;
+    double* post = nullptr;  // adjoint, line path: the multiplied pair carried to the next step
+    const double* aux = nullptr;
+    std::function<double*(int)> out;        // destination of pair j
+    std::function<const double*(int)> fwd;  // forward pair j
+};
+int run_lockstep(lfts_ctx* c, int i0, int i1, ChainSpec* const* chains, int n);
+
+// out[0] = scale * sum_r x[r] y[r] (y null: 1), deterministic two-stage sum
+int dev_sum(lfts_ctx* c, const double* x, const double* y, long n, double scale, double* out);
+
+// checkpointed storage (lfts_ckpt.hip)
+int ckpt_calc_concs(lfts_ctx* c);
+int ckpt_derived(lfts_ctx* c, int mode, const double* d1, const double* dstart1);
+int ckpt_tan_finish(lfts_ctx* c, const double* e, const double* dQ, double* dphi);
+int ckpt_adj_finish(lfts_ctx* c, const double* ubar, double* wbar);
+const double* ckpt_pair_N(lfts_ctx* c, bool derived);
+int set_pair_storage(lfts_ctx* c, int stride);
+void release_pair_storage(lfts_ctx* c);
 int dif_pos(int k, const std::vector<int>& rad, size_t s, int B);
 
 }  // namespace lfts_abi
