@@ -32,6 +32,7 @@ if ROOT not in sys.path:
 METRIC = "Langevin steps/s at 64³,N=90 FP64 (plain vs batched); % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
 NK_STEPS = 10  # Langevin steps timed with the Newton-Krylov solver in the derivative leg
+PROF_STEPS = 3  # steps after the timed region with live event timing of the contour chains
 
 
 def log(*a):
@@ -247,6 +248,25 @@ def adjoint_leg(ctx, eta_dev, eta_more, XbN, dt, M, torch, tols=(1e-4, 1e-12)):
     t0 = time.perf_counter()
     ctx.box_derivative()
     t_box = time.perf_counter() - t0
+    # the same products with checkpointed pair storage (stride ceil(sqrt(N)))
+    stride = int(math.ceil(math.sqrt(ctx.N)))
+    ctx.set_pair_storage(stride)
+    kept = ctx.pair_storage()[1]
+    ctx.calc_concs()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.calc_concs()
+    t_pe_ck = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.jvp_dev(dev.data_ptr(), dev.data_ptr() + 8 * M, dev.data_ptr() + 8 * 3 * M)
+    t_jvp_ck = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.vjp_dev(dev.data_ptr() + 8 * M, dev.data_ptr() + 8 * 3 * M)
+    t_vjp_ck = (time.perf_counter() - t0) / reps
+    ctx.set_pair_storage(0)
     return {
         "workload": "64^3 N=90 state after the timed steps plus one W- update; Anderson (10 histories) and "
                     "Newton-Krylov (GMRES(20), homogeneous-melt preconditioner) from the same fields",
@@ -256,6 +276,14 @@ def adjoint_leg(ctx, eta_dev, eta_more, XbN, dt, M, torch, tols=(1e-4, 1e-12)):
         "jvp_with_forward_ms": 1e3 * t_jvp,
         "vjp_with_forward_ms": 1e3 * t_vjp,
         "box_derivative_ms": 1e3 * t_box,
+        "checkpointed": {
+            "stride": stride,
+            "pairs_kept": kept,
+            "pairs_full": ctx.N,
+            "calc_concs_ms": 1e3 * t_pe_ck,
+            "jvp_with_forward_ms": 1e3 * t_jvp_ck,
+            "vjp_with_forward_ms": 1e3 * t_vjp_ck,
+        },
     }
 
 
@@ -302,7 +330,7 @@ def main():
     log(f"initial mix: k={k0} err={err0:.3e} ({time.perf_counter() - t0:.2f} s)")
 
     nsteps = args.warmup + args.steps
-    n_extra = 0 if args.no_adjoint else 1 + NK_STEPS
+    n_extra = PROF_STEPS + (0 if args.no_adjoint else 1 + NK_STEPS)
     eta = torch.empty((nsteps + n_extra, M), dtype=torch.float64, device="cuda")
     for s in range(nsteps + n_extra):
         eta[s].copy_(torch.from_numpy(rng.normals(M)))
@@ -313,7 +341,6 @@ def main():
         log(f"warmup {s}: lnQ={lnq:.10g} n_AM={k - 1}")
     torch.cuda.synchronize()
 
-    ctx.profile(True)
     nAM = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -322,10 +349,15 @@ def main():
         nAM.append(k - 1)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps: {elapsed:.3f} s, n_AM={nAM}, lnQ={lnq:.10g}")
+    # a few more steps with live HIP-event timing of the contour chains (the roofline
+    # figure), outside the timed region
+    ctx.profile(True)
+    for s in range(nsteps, nsteps + PROF_STEPS):
+        ctx.full_step_dev(eta[s].data_ptr(), XbN, dt)
     ms_chain, n_cs, ms_pe, n_pe = ctx.profile_read()
     ctx.profile(False)
     tuning_plain = ctx.tuning().strip()
-    log(f"timed {args.steps} steps: {elapsed:.3f} s, n_AM={nAM}, lnQ={lnq:.10g}")
 
     steps_per_s = args.steps / elapsed
     cs_ms = ms_chain / max(n_cs, 1)
@@ -344,8 +376,8 @@ def main():
 
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_leg(ctx, eta[nsteps].data_ptr(), [eta[nsteps + 1 + k] for k in range(NK_STEPS)], XbN, dt,
-                              M, torch)
+        e0 = nsteps + PROF_STEPS
+        adjoint = adjoint_leg(ctx, eta[e0].data_ptr(), [eta[e0 + 1 + k] for k in range(NK_STEPS)], XbN, dt, M, torch)
         log(f"adjoint leg: {adjoint}")
 
     batched = None
@@ -381,7 +413,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": measured_traffic(args.mesh),
-            "kernel": "contour step (48*M algorithmic bytes), HIP events on the context stream",
+            "kernel": "contour step (48*M algorithmic bytes), HIP events on the context stream over "
+                      f"{PROF_STEPS} further steps",
             "contour_step_us": cs_ms * 1e3,
             "slab_tuning": tuning_plain,
         },

@@ -22,10 +22,12 @@ DRIVER = os.path.join(BIN_DIR, "lfts-gpu")
 VTK_TOOL = os.path.join(BIN_DIR, "lfts-vtk")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-# FP64 throughout; device code may contract a*b+c into FMA (about 1 ulp from the
-# reference's separate multiply and add). Packed-FP32 instructions are disabled.
+# FP64 throughout; device code may contract a*b+c within one expression into an FMA
+# (about 1 ulp from the reference's separate multiply and add). "on" rather than "fast"
+# so that `#pragma clang fp contract(off)` holds where the rounding is spelled out (the
+# slab pass, the device Anderson solve). Packed-FP32 instructions are disabled.
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=fast",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=on",
     "-Wno-unused-value", "-Wno-pass-failed",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
@@ -40,11 +42,30 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError("build failed: " + " ".join(cmd[:3]) + " ...")
 
 
+# Per-file contraction: the slab pass is built with "fast" (the compiler fuses the kernel
+# multiplies into the following butterflies; every kernel shape still rounds alike, which
+# the shape tests check bit for bit).
+FILE_FLAGS = {"lfts_fused.hip": ["-ffp-contract=fast"]}
+
+
 def build_library() -> str:
+    """Compile every source to an object (in parallel), then link the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(LIB_DIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in LIB_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    obj_dir = os.path.join(LIB_DIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    srcs = [s for s in LIB_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objs = [os.path.join(obj_dir, s.replace(".hip", ".o")) for s in srcs]
+
+    def compile_one(i):
+        _run([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(srcs[i], []), "-c", "-I",
+              os.path.join(ROOT, "include"), "-o", objs[i], os.path.join(CSRC, srcs[i])])
+
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        list(ex.map(compile_one, range(len(srcs))))
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wno-unused-command-line-argument", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

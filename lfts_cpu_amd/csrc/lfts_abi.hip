@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <fstream>
 #include <numeric>
@@ -150,14 +151,18 @@ double* pair_ptr(lfts_ctx* c, int i) { return (i == 1) ? c->h : c->q + 2L * (i -
 int line_contour_step(lfts_ctx* c, const double* in, double* out, const double* h, int i, double* tmp,
                       hipStream_t st) {
     const long M = c->M;
+    const int* skip = c->self ? c->self->reps[0].skip : nullptr;
     LineArgs z = line_args(c, 2, lfts::LINE_CONV);
     z.inA = in; z.inB = in + M; z.outA = tmp; z.outB = tmp + M;
+    z.skip = skip;
     HIPCHK(c, lfts::launch_line(z, st));
     LineArgs y = line_args(c, 1, lfts::LINE_CONV);
     y.inA = tmp; y.inB = tmp + M; y.outA = tmp; y.outB = tmp + M;
+    y.skip = skip;
     HIPCHK(c, lfts::launch_line(y, st));
     LineArgs x = line_args(c, 0, lfts::LINE_CONV);
     x.inA = tmp; x.inB = tmp + M; x.outA = out; x.outB = out + M;
+    x.skip = skip;
     x.hA = (i < c->NA) ? h : h + M;       // q_{i+1}   (src/step.cc:53-57)
     x.hB = (i < c->NB) ? h + M : h;       // q+_{N-i} (src/step.cc:60-64)
     HIPCHK(c, lfts::launch_line(x, st));
@@ -439,6 +444,7 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             lfts_ctx* c = b->ctx[id];
             Coef& co = b->h_coef[id];
             co.nh = (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax;
+            co.nh_dots = co.nh;
             nh_max = std::max(nh_max, co.nh);
         }
         if (int rc = upload_coef(b)) return rc;
@@ -496,6 +502,121 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
         if (!std::isfinite(err[id])) nonfinite = true;
     }
     return nonfinite ? bfail(b, LFTS_ENUM, "Anderson residual is not finite") : LFTS_OK;
+}
+
+// ---- Anderson mixing on the device, one context -----------------------------------------
+
+// The device-resident loop is chosen with LFTS_ANDERSON=device. It is not the default:
+// at 64^3, N=90 the host loop (one synchronisation and a host LU per iteration) measured
+// 48.2 ms per Langevin step against 49.6 ms for the device loop (its two extra kernels
+// per iteration, the flag test in every kernel and the speculative tail cost more than the
+// round trip saves); graph replay of an iteration made no measurable difference.
+bool device_anderson_enabled(const lfts_ctx* c) {
+    const char* e = std::getenv("LFTS_ANDERSON");
+    return !c->ck_stride && e && std::strcmp(e, "device") == 0;
+}
+
+int ensure_am_device(lfts_ctx* c, int maxIter) {
+    if (!c->am) HIPCHK(c, hipMalloc(&c->am, sizeof(lfts::AmState)));
+    if (!c->am_h) HIPCHK(c, hipHostMalloc(&c->am_h, sizeof(lfts::AmState), hipHostMallocDefault));
+    if (c->pow09_n < maxIter + 2) {
+        // lambda = 1 - 0.9^k with the host's pow (src/anderson.cc:41)
+        std::vector<double> t(maxIter + 2);
+        for (int k = 0; k < maxIter + 2; k++) t[k] = std::pow(0.9, double(k));
+        (void)hipFree(c->pow09);
+        c->pow09 = nullptr;
+        HIPCHK(c, hipMalloc(&c->pow09, sizeof(double) * t.size()));
+        HIPCHK(c, hipMemcpy(c->pow09, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+        c->pow09_n = maxIter + 2;
+    }
+    lfts_batch* b = c->self;
+    const int* flag = &c->am->stop;
+    if (b->reps[0].skip != flag) {
+        b->reps[0].skip = flag;  // the set's kernels return at once while the flag is set
+        if (c->am_exec) {
+            (void)hipGraphExecDestroy(c->am_exec);
+            (void)hipGraphDestroy(c->am_graph);
+            c->am_exec = nullptr;
+            c->am_graph = nullptr;
+        }
+    }
+    return LFTS_OK;
+}
+
+// One iteration of src/anderson.cc:29-89 as device work only: propagator evaluation,
+// Gram row, solve, update, loop test.
+int am_iteration(lfts_ctx* c) {
+    lfts_batch* b = c->self;
+    if (int rc = calc_concs_async(b)) return rc;
+    BHIPCHK(b, lfts::launch_am_dots(b->d_ract, 1, b->nb_max, b->d_coef, b->d_part, b->st));
+    BHIPCHK(b, lfts::launch_reduce(b->d_ract, 1, b->d_part, c->nhMax + 1, 0, b->d_scal, 0, b->st));
+    BHIPCHK(b, lfts::launch_am_solve(c->am, b->d_scal, b->d_coef, c->pow09, c->M, c->nhMax, b->st));
+    BHIPCHK(b, lfts::launch_am_update(b->d_ract, 1, b->nb_max, b->d_coef, b->st));
+    BHIPCHK(b, lfts::launch_am_advance(c->am, b->st));
+    return LFTS_OK;
+}
+
+// anderson::mix with the loop on the device: iterations are enqueued in chunks (the size
+// of the previous solve first), and those enqueued past the end of the loop return at
+// once. With LFTS_GRAPH unset or 1 (and no live profiling), one iteration is captured as
+// a HIP graph on first use and replayed.
+int device_anderson_mix(lfts_ctx* c, int maxIter, double errTol, int* k_ret, double* err_ret) {
+    lfts_batch* b = c->self;
+    if (int rc = ensure_am_device(c, maxIter)) return rc;
+    if (int rc = set_active(b, all_ids(b))) return rc;
+    lfts::AmState* h = c->am_h;
+    std::memset(h, 0, sizeof(lfts::AmState));
+    h->k = 1;
+    h->maxIter = maxIter;
+    h->err = 1.0;
+    h->tol = errTol;
+    h->stop = !(1 < maxIter && 1.0 > errTol);
+    BHIPCHK(b, hipMemcpyAsync(c->am, h, sizeof(lfts::AmState), hipMemcpyHostToDevice, b->st));
+    b->h_coef[0].nh_dots = 0;
+    if (int rc = upload_coef(b)) return rc;
+    const char* ge = std::getenv("LFTS_GRAPH");
+    const bool graph = !(ge && std::strcmp(ge, "0") == 0) && !b->prof;
+    const long long pe0 = c->n_pe, cs0 = c->n_cs;
+    int enq = 0;
+    bool first = true;
+    while (true) {
+        const int n = first ? std::max(1, c->am_last > 2 ? c->am_last - 2 : 6) : 2;
+        for (int t = 0; t < n; t++) {
+            if (graph && c->am_exec) {
+                BHIPCHK(b, hipGraphLaunch(c->am_exec, b->st));
+            } else if (graph && enq > 0) {
+                // capture after one direct iteration (kernel shapes are tuned by then)
+                BHIPCHK(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+                const int rc = am_iteration(c);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(b->st, &g);
+                if (rc) return rc;
+                BHIPCHK(b, e);
+                c->am_graph = g;
+                BHIPCHK(b, hipGraphInstantiate(&c->am_exec, g, nullptr, nullptr, 0));
+                BHIPCHK(b, hipGraphLaunch(c->am_exec, b->st));
+            } else {
+                if (int rc = am_iteration(c)) return rc;
+            }
+            enq++;
+        }
+        first = false;
+        BHIPCHK(b, hipMemcpyAsync(h, c->am, offsetof(lfts::AmState, DD), hipMemcpyDeviceToHost, b->st));
+        if (int rc = sync_batch(b)) return rc;
+        if (h->stop || enq > maxIter + 2) break;
+    }
+    const int k = h->k;
+    const double err = h->err;
+    // the flag back to 0 so that later evaluations run
+    h->stop = 0;
+    BHIPCHK(b, hipMemcpyAsync(&c->am->stop, &h->stop, sizeof(int), hipMemcpyHostToDevice, b->st));
+    if (int rc = sync_batch(b)) return rc;
+    c->am_last = k - 1;
+    c->n_pe = pe0 + (k - 1);
+    c->n_cs = cs0 + (long long)(k - 1) * (c->N - 1);
+    if (k_ret) *k_ret = k;
+    if (err_ret) *err_ret = err;
+    return std::isfinite(err) ? LFTS_OK : fail(c, LFTS_ENUM, "Anderson residual is not finite");
 }
 
 // src/langevin.cc:23-37 for every replica; eta_new are device pointers.
@@ -632,6 +753,11 @@ int lfts_destroy(lfts_ctx* c) {
     if (!c) return LFTS_EINVAL;
     if (c->self) lfts_batch_destroy(c->self);
     release_pair_storage(c);
+    if (c->am_exec) (void)hipGraphExecDestroy(c->am_exec);
+    if (c->am_graph) (void)hipGraphDestroy(c->am_graph);
+    (void)hipFree(c->am);
+    (void)hipFree(c->pow09);
+    if (c->am_h) (void)hipHostFree(c->am_h);
     for (auto& p : c->ax) {
         (void)hipFree(p.tw);
         (void)hipFree(p.gperm);
@@ -683,6 +809,7 @@ int lfts_calc_concs(lfts_ctx* c, double* lnQ) {
 
 int lfts_anderson_mix(lfts_ctx* c, int maxIter, double errTol, int* k_ret, double* err_ret) {
     if (!c) return LFTS_EINVAL;
+    if (device_anderson_enabled(c)) return device_anderson_mix(c, maxIter, errTol, k_ret, err_ret);
     return batch_anderson_mix(c->self, maxIter, errTol, k_ret, err_ret);
 }
 
@@ -710,7 +837,15 @@ int lfts_langevin_step_dev(lfts_ctx* c, const double* eta_new_dev, double XbN, d
 int lfts_langevin_full_step_dev(lfts_ctx* c, const double* eta_new_dev, double XbN, double dt, int maxIter,
                                 double errTol, double* lnQ, int* k_ret) {
     if (!c) return LFTS_EINVAL;
-    return lfts_batch_full_step_dev(c->self, &eta_new_dev, &XbN, &dt, maxIter, errTol, lnQ, k_ret);
+    if (!device_anderson_enabled(c))
+        return lfts_batch_full_step_dev(c->self, &eta_new_dev, &XbN, &dt, maxIter, errTol, lnQ, k_ret);
+    if (!eta_new_dev) return fail(c, LFTS_EINVAL, "full_step_dev: null argument");
+    int rc = batch_langevin(c->self, &eta_new_dev, &XbN, &dt);
+    if (rc) return rc;
+    rc = device_anderson_mix(c, maxIter, errTol, k_ret, nullptr);
+    if (rc && rc != LFTS_ENUM) return rc;
+    const int rc2 = batch_calc_concs(c->self, lnQ);
+    return rc ? rc : rc2;
 }
 
 // src/strFunc.cc:36-46 (forward transform of W- in three line passes, |X|^2 accumulated

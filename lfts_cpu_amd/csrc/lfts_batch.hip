@@ -42,7 +42,7 @@ __device__ __forceinline__ double* part_row(double* part, int slot) {
 #define FOR_POINTS(r, R) for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < (R).M; r += (long)(R).nb * blockDim.x)
 #define REPLICA(R)                              \
     const Rep R = reps[blockIdx.y];             \
-    if ((int)blockIdx.x >= R.nb) return
+    if ((int)blockIdx.x >= R.nb || (R.skip && *R.skip)) return
 
 }  // namespace
 
@@ -73,6 +73,7 @@ __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, co
     __shared__ double red[8];
     const int slot = blockIdx.x;
     const Rep R = reps[slot];
+    if (R.skip && *R.skip) return;
     const double* p = part + (long)slot * MAX_NB * PSTRIDE;
     for (int c = 0; c < ncomp; c++) {
         double s = 0.0;
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(256) void k_am_dots(const Rep* __restrict__ reps, c
                                                  double* __restrict__ part) {
     __shared__ double red[8];
     REPLICA(R);
-    const int nh = coef[R.id].nh;
+    const int nh = coef[R.id].nh_dots;
     const long M = R.M;
     double acc[MAX_HIST + 1];
 #pragma unroll
@@ -180,6 +181,108 @@ __global__ __launch_bounds__(256) void k_h_sums(const Rep* __restrict__ reps, do
     }
 }
 
+// The body of src/anderson.cc:29-89 after the Gram row: on one thread, in the order of
+// the host loop (the LU is the host's lu_solve, same operations in the same order). The
+// history Gram matrix, the row and the LU work arrays live in LDS, so the serial part
+// touches no global memory until the results are written back.
+#pragma clang fp contract(off)
+__global__ __launch_bounds__(64) void k_am_solve(AmState* __restrict__ s, const double* __restrict__ scal,
+                                                 Coef* __restrict__ coef, const double* __restrict__ pow09, long M,
+                                                 int nhMax) {
+    constexpr int HM = MAX_HIST + 1;
+    __shared__ double DD[HM * HM];
+    __shared__ double U[MAX_HIST * MAX_HIST], V[MAX_HIST], y[MAX_HIST], C[MAX_HIST];
+    __shared__ int p[MAX_HIST];
+    __shared__ int hdr[2];
+    const int H = nhMax + 1;
+    if (threadIdx.x == 0) {
+        hdr[0] = s->stop;
+        hdr[1] = s->k;
+    }
+    __syncthreads();
+    if (hdr[0]) return;
+    const int k = hdr[1];
+    const int nh = (k < nhMax + 1) ? k - 1 : nhMax;
+    for (int i = threadIdx.x; i < H * H; i += blockDim.x) DD[i] = s->DD[i];
+    __syncthreads();
+    for (int m = threadIdx.x; m <= nh; m += blockDim.x) DD[m] = scal[m];  // DD(0, m)
+    __syncthreads();
+    if (threadIdx.x != 0) goto writeback;
+    {
+        const double err = sqrt(DD[0] / (double)M);
+        const double lambda = 1.0 - pow09[k];
+        int nh_upd = 0;
+        if (k >= 2) {
+            for (int m = 1; m <= nh; m++) {
+                V[m - 1] = DD[0] - DD[m];
+                for (int n = 1; n <= m; n++)
+                    U[(m - 1) * nh + n - 1] = U[(n - 1) * nh + m - 1] = DD[0] - DD[m] - DD[n] + DD[n * H + m];
+            }
+            // LU with partial pivoting, then the two triangular solves
+            for (int i = 0; i < nh; i++) p[i] = i;
+            for (int j = 0; j < nh; j++) {
+                int ip = j;
+                double amax = fabs(U[j * nh + j]);
+                for (int i = j + 1; i < nh; i++) {
+                    const double v = fabs(U[i * nh + j]);
+                    if (v > amax) {
+                        amax = v;
+                        ip = i;
+                    }
+                }
+                if (ip != j) {
+                    for (int kk = 0; kk < nh; kk++) {
+                        const double t = U[j * nh + kk];
+                        U[j * nh + kk] = U[ip * nh + kk];
+                        U[ip * nh + kk] = t;
+                    }
+                    const int t = p[j];
+                    p[j] = p[ip];
+                    p[ip] = t;
+                }
+                const double ajj = U[j * nh + j];
+                if (ajj == 0.0) continue;
+                for (int i = j + 1; i < nh; i++) {
+                    const double l = U[i * nh + j] / ajj;
+                    U[i * nh + j] = l;
+                    for (int kk = j + 1; kk < nh; kk++) U[i * nh + kk] -= l * U[j * nh + kk];
+                }
+            }
+            for (int i = 0; i < nh; i++) {
+                double t = V[p[i]];
+                for (int kk = 0; kk < i; kk++) t -= U[i * nh + kk] * y[kk];
+                y[i] = t;
+            }
+            for (int i = nh - 1; i >= 0; i--) {
+                double t = y[i];
+                for (int kk = i + 1; kk < nh; kk++) t -= U[i * nh + kk] * C[kk];
+                C[i] = t / U[i * nh + i];
+            }
+            nh_upd = nh;
+        }
+        const int slot = 1 + (k - 1) % nhMax;
+        DD[slot * H + slot] = DD[0];
+        for (int m = 1; m < slot; m++) DD[m * H + slot] = DD[m];
+        for (int m = slot + 1; m <= nh; m++) DD[slot * H + m] = DD[m];
+        Coef& c = coef[0];
+        c.lambda = lambda;
+        c.nh = nh_upd;
+        c.slot = slot;
+        for (int i = 0; i < nh_upd; i++) c.C[i] = C[i];
+        c.nh_dots = (k + 1 < nhMax + 1) ? k : nhMax;
+        s->err = err;
+        s->k = k + 1;
+        s->stop_next = !(k + 1 < s->maxIter && err > s->tol);
+    }
+writeback:
+    __syncthreads();
+    for (int i = threadIdx.x; i < H * H; i += blockDim.x) s->DD[i] = DD[i];
+}
+
+__global__ void k_am_advance(AmState* __restrict__ s) {
+    if (threadIdx.x == 0 && !s->stop) s->stop = s->stop_next;
+}
+
 #define LAUNCH(kernel, grid, ...)                                            \
     do {                                                                     \
         hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, __VA_ARGS__);     \
@@ -210,6 +313,16 @@ hipError_t launch_langevin(const Rep* reps, int nact, int nb_max, const Coef* co
 }
 hipError_t launch_h_sums(const Rep* reps, int nact, int nb_max, double* part, hipStream_t st) {
     LAUNCH(k_h_sums, dim3(nb_max, nact), reps, part);
+}
+
+hipError_t launch_am_solve(AmState* s, const double* scal, Coef* coef, const double* pow09, long M, int nhMax,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_am_solve, dim3(1), dim3(64), 0, st, s, scal, coef, pow09, M, nhMax);
+    return hipGetLastError();
+}
+hipError_t launch_am_advance(AmState* s, hipStream_t st) {
+    hipLaunchKernelGGL(k_am_advance, dim3(1), dim3(64), 0, st, s);
+    return hipGetLastError();
 }
 
 }  // namespace lfts

@@ -242,6 +242,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     constexpr int TBYTES = TWL ? E * 16 + 2 * E * 8 : 0;
     constexpr bool EXTRA = MODE != CHAIN_PLAIN;
     __shared__ __align__(16) unsigned char smem[FPW * BYTES + TBYTES];
+    if (io.skip && *io.skip) return;
 
     const bool xslab = io.xslab != 0;
     const bool finish = io.finish != 0, cont = io.cont != 0;

@@ -43,12 +43,14 @@ struct Rep {
     int n;              // cubic edge when the fused slab path applies, else 0
     int id;             // index of the replica in its set (scalar results, coefficients)
     int nb;             // workgroups of the element-wise passes, blocks_for(M)
+    const int* skip;    // device flag: when set, this replica's kernels return at once (or null)
 };
 
 // Per-replica coefficients of one Anderson update or Langevin step.
 struct Coef {
     double lambda;
     int nh, slot;
+    int nh_dots;          // history depth of the next Gram row (k_am_dots)
     double C[MAX_HIST];
     double XbN, dt;
     const double* eta_new;
@@ -87,6 +89,7 @@ struct LineArgs {
     // the line's transform order, mul[line * n + e]
     double* outR;
     const double* mul;
+    const int* skip;      // device flag: when set, the launch returns at once (or null)
 };
 hipError_t launch_line(const LineArgs& a, hipStream_t st);
 
@@ -105,6 +108,7 @@ struct SlabIO {
     const double* aux[2];  // tangent: e per field; adjoint: alpha per field
     const double* src[2];  // the stored values aux multiplies, per field
     const double* g[3];    // per-axis kernels, natural frequency order
+    const int* skip;       // device flag: when set, the pass returns at once (or null)
     long M;
     int finish, cont, xslab;
 };
@@ -120,6 +124,7 @@ __host__ __device__ inline SlabIO plain_io(const Rep& R, int i) {
     io.h[0] = (i < R.NA) ? R.h : R.h + M;
     io.h[1] = (i < R.NB) ? R.h + M : R.h;
     for (int d = 0; d < 3; d++) io.g[d] = R.g[d];
+    io.skip = R.skip;
     io.M = M;
     io.finish = i >= 1;
     io.cont = i < R.N - 1;
@@ -153,5 +158,23 @@ hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coe
 hipError_t launch_am_update(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st);
 hipError_t launch_langevin(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st);
 hipError_t launch_h_sums(const Rep* reps, int nact, int nb_max, double* part, hipStream_t st);
+
+// ---- Anderson mixing on the device (one replica) ----------------------------------------
+// The loop state of anderson::mix (src/anderson.cc:24-93) kept in device memory, so that
+// iterations can be enqueued ahead of the loop test: every kernel of an iteration returns
+// at once when `stop` is set.
+struct AmState {
+    int k;             // iteration number (k of src/anderson.cc:29)
+    int stop;          // loop condition failed before this iteration
+    int stop_next;     // loop condition for the next iteration
+    int maxIter;
+    double err, tol;   // residual of the last iteration, tolerance
+    double DD[(MAX_HIST + 1) * (MAX_HIST + 1)];
+};
+// after an iteration's Gram row landed in scal[0..nh]: residual, coefficients (LU on one
+// thread, the host LU's operation order), history bookkeeping, next loop test
+hipError_t launch_am_solve(AmState* s, const double* scal, Coef* coef, const double* pow09, long M, int nhMax,
+                           hipStream_t st);
+hipError_t launch_am_advance(AmState* s, hipStream_t st);
 
 }  // namespace lfts

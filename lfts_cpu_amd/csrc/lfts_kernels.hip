@@ -138,6 +138,7 @@ __device__ __forceinline__ long line_base(const LineArgs& a, long line) {
 //  mode LINE_CMUL: complex in -> FFT -> *mul (per point) -> IFFT -> complex out.
 __global__ __launch_bounds__(256) void k_line(LineArgs a) {
     extern __shared__ double2 lds[];
+    if (a.skip && *a.skip) return;
     const int n = a.n, T = a.T, pitch = a.pitch;
     const long line0 = (long)blockIdx.x * T;
     const bool zaxis = (a.axis == 2);

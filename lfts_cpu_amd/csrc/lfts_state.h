@@ -65,6 +65,15 @@ struct lfts_ctx {
     double* kry = nullptr;    // (kry_m + 1) M Krylov basis, then 5M work vectors
     int kry_m = 0;
     double* pmul = nullptr;   // M, preconditioner multiplier in the x-line transform order
+    // Anderson mixing on the device (single context): loop state, (0.9)^k table, the
+    // captured graph of one iteration
+    lfts::AmState* am = nullptr;
+    lfts::AmState* am_h = nullptr;  // pinned
+    double* pow09 = nullptr;
+    int pow09_n = 0;
+    int am_last = 0;                // iterations of the previous solve (chunk size)
+    hipGraph_t am_graph = nullptr;
+    hipGraphExec_t am_exec = nullptr;
     int nsamples = 0;
     long long n_pe = 0, n_cs = 0, n_tan = 0, n_adj = 0;
     std::vector<double> DD, U, V, C;  // Anderson host state (src/anderson.cc:8-21)
