@@ -161,6 +161,10 @@ int lfts_batch_full_step_dev(lfts_batch* b, const double* const* eta_new_dev, co
  * to 0..3 to fix the shape instead. The text stays valid until the next call on the set. */
 const char* lfts_batch_tuning(const lfts_batch* b);
 const char* lfts_tuning(const lfts_ctx* ctx);
+/* lfts_batch_full_step_dev with host noise: member id's M doubles eta_new[id] are copied
+ * to its device staging buffer first (as lfts_langevin_step does for one context). */
+int lfts_batch_full_step(lfts_batch* b, const double* const* eta_new, const double* XbN, const double* dt,
+                         int maxIter, double errTol, double* lnQ, int* k_ret);
 int lfts_batch_profile(lfts_batch* b, int enable);
 int lfts_batch_profile_read(lfts_batch* b, double* ms_chain, long long* n_passes, double* ms_pe,
                             long long* n_prop_evals);

@@ -15,6 +15,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "liblfts_gpu.so")
 DRIVER_PATH = os.path.join(PKG, "bin", "lfts-gpu")
 VTK_PATH = os.path.join(PKG, "bin", "lfts-vtk")
+BATCH_DRIVER_PATH = os.path.join(PKG, "bin", "lfts-gpu-batch")
 
 LFTS_OK, LFTS_EINVAL, LFTS_EHIP, LFTS_ESTATE, LFTS_ENUM = range(5)
 
@@ -109,6 +110,8 @@ SIGNATURES = {
                                                 _dp, _ip]),
     "lfts_batch_tuning": (ctypes.c_char_p, [_vp]),
     "lfts_tuning": (ctypes.c_char_p, [_vp]),
+    "lfts_batch_full_step": (ctypes.c_int, [_vp, ctypes.POINTER(_dp), _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                            _dp, _ip]),
     "lfts_batch_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lfts_batch_profile_read": (ctypes.c_int, [_vp, _dp, _llp, _dp, _llp]),
     "lfts_rng_create": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_double, ctypes.POINTER(_vp)]),

@@ -3,6 +3,7 @@
 Outputs:
   lfts_cpu_amd/lib/liblfts_gpu.so   C-ABI library (kernels + host control)
   lfts_cpu_amd/bin/lfts-gpu         drop-in driver for `lfts-cpu <inputFile>`
+  lfts_cpu_amd/bin/lfts-gpu-batch   many inputs advanced together as one replica set
   lfts_cpu_amd/bin/lfts-vtk         field file -> VTK (the reference's tools/make_vtk.sh)
 """
 from __future__ import annotations
@@ -19,6 +20,7 @@ LIB_DIR = os.path.join(PKG, "lib")
 BIN_DIR = os.path.join(PKG, "bin")
 LIB = os.path.join(LIB_DIR, "liblfts_gpu.so")
 DRIVER = os.path.join(BIN_DIR, "lfts-gpu")
+BATCH_DRIVER = os.path.join(BIN_DIR, "lfts-gpu-batch")
 VTK_TOOL = os.path.join(BIN_DIR, "lfts-vtk")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -72,11 +74,14 @@ def build_library() -> str:
 
 def build_driver() -> str:
     os.makedirs(BIN_DIR, exist_ok=True)
-    srcs = [os.path.join(HOST, s) for s in DRIVER_SOURCES]
-    tmp = DRIVER + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs,
-          "-L", LIB_DIR, "-llfts_gpu", "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"])
-    os.replace(tmp, DRIVER)
+    for out, main in ((DRIVER, "fts_gpu.cc"), (BATCH_DRIVER, "fts_gpu_batch.cc")):
+        srcs = [os.path.join(HOST, s) for s in DRIVER_SOURCES if s != "fts_gpu.cc"] + [os.path.join(HOST, main)]
+        if main != "fts_gpu.cc":
+            srcs = [os.path.join(HOST, "lfts_params.cc"), os.path.join(HOST, main)]
+        tmp = out + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs,
+              "-L", LIB_DIR, "-llfts_gpu", "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"])
+        os.replace(tmp, out)
     return DRIVER
 
 

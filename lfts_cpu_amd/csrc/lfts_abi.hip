@@ -1089,6 +1089,20 @@ const char* lfts_batch_tuning(const lfts_batch* b) { return b ? b->tune_report.c
 
 const char* lfts_tuning(const lfts_ctx* c) { return (c && c->self) ? c->self->tune_report.c_str() : ""; }
 
+int lfts_batch_full_step(lfts_batch* b, const double* const* eta_new, const double* XbN, const double* dt, int maxIter,
+                         double errTol, double* lnQ, int* k_ret) {
+    if (!b || !eta_new || !XbN || !dt) return bfail(b, LFTS_EINVAL, "full_step: null argument");
+    const int n = (int)b->ctx.size();
+    std::vector<const double*> dev(n);
+    for (int id = 0; id < n; id++) {
+        lfts_ctx* c = b->ctx[id];
+        if (!eta_new[id]) return bfail(b, LFTS_EINVAL, "full_step: null noise pointer");
+        dev[id] = c->noise + c->M;  // the member's staging buffer
+        BHIPCHK(b, hipMemcpyAsync(c->noise + c->M, eta_new[id], sizeof(double) * c->M, hipMemcpyHostToDevice, b->st));
+    }
+    return lfts_batch_full_step_dev(b, dev.data(), XbN, dt, maxIter, errTol, lnQ, k_ret);
+}
+
 int lfts_batch_profile(lfts_batch* b, int enable) {
     if (!b) return LFTS_EINVAL;
     if (int rc = sync_batch(b)) return rc;

@@ -78,3 +78,39 @@ def test_vtk_export_matches_the_converter(gpu, tmp_path):
         r = subprocess.run([L.VTK_PATH, f, "12", "12", "12"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode == 0
         assert (tmp_path / (f + ".vtk")).read_bytes() == got, f
+
+
+def test_replica_set_driver_matches_single_runs(gpu, tmp_path):
+    """`lfts-gpu-batch` advances several inputs as one replica set; every replica's files
+    and lnQ lines equal those of `lfts-gpu` run on that input alone."""
+    specs = [("16 8 14.0 50 0.5 1\n12 12 12 3.0 3.0 3.0\n", 12 ** 3, 41),
+             ("12 5 12.0 40 0.4 0\n10 8 6 3.1 2.6 2.2\n", 10 * 8 * 6, 42),
+             ("10 4 15.0 50 0.5 1\n32 32 32 4.0 4.0 4.0\n", 32 ** 3, 43)]
+    tail = "4 4 2 2 1\n"
+    inputs, singles = [], []
+    for r, (head, M, seed) in enumerate(specs):
+        rng = np.random.default_rng(seed)
+        w0 = np.concatenate([rng.normal(0, 0.1, M), rng.normal(0, 0.1, M)])
+        text = head + tail + "".join("%.17g\n" % v for v in w0)
+        d = tmp_path / f"rep{r}"
+        d.mkdir()
+        (d / "input").write_text(text)
+        inputs.append(d / "input")
+        s = tmp_path / f"single{r}"
+        s.mkdir()
+        (s / "input").write_text(text)
+        singles.append(_run([s / "input"], s))
+    print("batch driver", flush=True)
+    rb = subprocess.run([L.BATCH_DRIVER_PATH, *map(str, inputs)], cwd=tmp_path, capture_output=True, text=True,
+                        timeout=300)
+    assert rb.returncode == 0, rb.stdout + rb.stderr
+    for r in range(len(specs)):
+        lb = [ln.split("] ", 1)[1] for ln in rb.stdout.splitlines() if ln.startswith(f"[{r}] lnQ = ")]
+        assert lb == _lnq(singles[r])
+        hb = [ln.split("= ")[1] for ln in rb.stdout.splitlines() if ln.startswith(f"[{r}] H = ")]
+        assert hb == [singles[r].strip().splitlines()[-1]]
+        da, db = tmp_path / f"single{r}", tmp_path / f"rep{r}"
+        files = sorted(os.listdir(da))
+        assert sorted(os.listdir(db)) == files
+        for f in files:
+            assert (da / f).read_bytes() == (db / f).read_bytes(), (r, f)
