@@ -313,20 +313,26 @@ int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep*
             if (!e) BHIPCHK(b, hipEventCreate(&e));
         const Rep* one = (cnt == 1) ? h_reps : nullptr;
         BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 0, tw, s, one, 0));
+        // three rounds over the variants in turn, the fastest round of each counts
         float t[lfts::SLAB_VARIANTS];
-        for (int v = 0; v < lfts::SLAB_VARIANTS; v++) {
-            BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 1, tw, s, one, v));  // warm-up
-            BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 2, tw, s, one, v));
-            BHIPCHK(b, hipEventRecord(b->tune_ev[0], s));
-            for (int rep = 0; rep < 4; rep++) {
-                BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 1, tw, s, one, v));
+        for (int v = 0; v < lfts::SLAB_VARIANTS; v++) t[v] = 1e30f;
+        for (int round = 0; round < 3; round++)
+            for (int v = 0; v < lfts::SLAB_VARIANTS; v++) {
+                BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 1, tw, s, one, v));  // warm-up
                 BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 2, tw, s, one, v));
+                BHIPCHK(b, hipEventRecord(b->tune_ev[0], s));
+                for (int rep = 0; rep < 4; rep++) {
+                    BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 1, tw, s, one, v));
+                    BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 2, tw, s, one, v));
+                }
+                BHIPCHK(b, hipEventRecord(b->tune_ev[1], s));
+                BHIPCHK(b, hipEventSynchronize(b->tune_ev[1]));
+                float ms = 0.f;
+                BHIPCHK(b, hipEventElapsedTime(&ms, b->tune_ev[0], b->tune_ev[1]));
+                t[v] = std::min(t[v], ms);
             }
-            BHIPCHK(b, hipEventRecord(b->tune_ev[1], s));
-            BHIPCHK(b, hipEventSynchronize(b->tune_ev[1]));
-            BHIPCHK(b, hipEventElapsedTime(&t[v], b->tune_ev[0], b->tune_ev[1]));
+        for (int v = 1; v < lfts::SLAB_VARIANTS; v++)
             if (t[v] < t[best]) best = v;
-        }
         std::snprintf(line, sizeof line, "edge %d x %d (tuned on %d): variant %d (us per pass: %.2f %.2f %.2f %.2f)\n",
                       edge, cls, cnt, best, 125.0 * t[0], 125.0 * t[1], 125.0 * t[2], 125.0 * t[3]);
     }
