@@ -16,14 +16,20 @@
 // read and written in the row-major layout with runs of >= 64 contiguous bytes.
 //
 // One workgroup owns one slab plane of one field (q or q+). The real plane is processed
-// as complex lines by pairing lines (column b with column b + n/2 for the a-axis, row a
-// with row a + n/2 for the z-axis); a 1D convolution with the real, even kernel maps
-// real lines to real lines, so real and imaginary parts separate exactly.
+// as complex lines by pairing lines (adjacent columns 2c, 2c+1 for the a-axis, so a lane
+// moves both with one 16-byte access; rows a and a + n/2 for the z-axis); a 1D
+// convolution with the real, even kernel maps real lines to real lines, so real and
+// imaginary parts separate exactly.
 //
 // A length-n line (n = R1*R2) is held by R2 threads with R1 elements each, thread j
 // holding x[j + R2 k]. Forward: radix-R1 DFT in registers, twiddle w_n^{jk'}, exchange
 // through LDS, radix-R2 DFTs -> X[k' + R1 f]; multiply by g; the inverse retraces the
 // steps with conjugate roots. The kernel factor g includes the 1/n of each inverse.
+// In LDS a line keeps element k*R2 + j at k*(R2+1) + j: the R2 threads of a line then
+// read their contiguous runs from distinct banks.
+//
+// Outputs (pairs and partial pairs) are written with non-temporal stores: they are read
+// next by workgroups on other XCDs, so they go to memory rather than stay in this L2.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -51,6 +57,14 @@ __device__ __forceinline__ double2 c_mulc(double2 a, double2 b) {  // a * conj(b
 template <bool INV>
 __device__ __forceinline__ double2 c_rot(double2 a) {
     return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
+__device__ __forceinline__ void st_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st2_nt(double* p, double2 v) {
+    const d2v w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<d2v*>(p));
 }
 
 template <bool INV>
@@ -183,18 +197,19 @@ struct LdsTab {
 template <int R1, int R2, class Tab>
 __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const Tab& tab) {
     constexpr int U = R1 / R2;
+    constexpr int Q = R2 + 1;  // padded run length in LDS
     dft<R1, false>(z);
 #pragma unroll
     for (int k = 1; k < R1; k++) z[k] = c_mul(z[k], tab.w(k));
     __syncthreads();  // previous readers of the exchange rows are done
 #pragma unroll
-    for (int k = 0; k < R1; k++) lx[k * R2 + j] = z[k];
+    for (int k = 0; k < R1; k++) lx[k * Q + j] = z[k];
     __syncthreads();
     double2 y[R1];
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int jj = 0; jj < R2; jj++) y[u * R2 + jj] = lx[(j * U + u) * R2 + jj];
+        for (int jj = 0; jj < R2; jj++) y[u * R2 + jj] = lx[(j * U + u) * Q + jj];
 #pragma unroll
     for (int u = 0; u < U; u++) {
         dft<R2, false>(y + u * R2);
@@ -210,10 +225,10 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int jj = 0; jj < R2; jj++) lx[(j * U + u) * R2 + jj] = y[u * R2 + jj];
+        for (int jj = 0; jj < R2; jj++) lx[(j * U + u) * Q + jj] = y[u * R2 + jj];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < R1; k++) z[k] = lx[k * R2 + j];
+    for (int k = 0; k < R1; k++) z[k] = lx[k * Q + j];
 #pragma unroll
     for (int k = 1; k < R1; k++) z[k] = c_mulc(z[k], tab.w(k));
     dft<R1, true>(z);
@@ -234,8 +249,8 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     static_assert(R1 * R2 == E && R1 % R2 == 0, "unsupported factorisation");
     constexpr int H = E / 2;           // complex lines per plane
     constexpr int TPF = H * R2;        // threads per field
-    constexpr int XP = E + 1;          // exchange row pitch (complex)
-    constexpr int RP = E + 1;          // real plane row pitch (doubles)
+    constexpr int XP = R1 * (R2 + 1) + 1;  // exchange row pitch (complex): padded line + 1
+    constexpr int RP = E + 8;          // real plane row pitch (doubles): rows r, r+1 on other banks
     constexpr int XBYTES = H * XP * 16;
     constexpr int RBYTES = E * RP * 8;
     constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
@@ -259,7 +274,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     double* rp = reinterpret_cast<double*>(smem + fl * BYTES);
     const long pbase = (long)plane * plane_stride;  // plane origin within a field
     const long base = (long)f * M + pbase;
-    const int c = t % H, j = t / H;    // layout A: a-lines, column pair (c, c + H), a = j + R2 k
+    const int c = t % H, j = t / H;    // layout A: a-lines, column pair (2c, 2c + 1), a = j + R2 k
     const int r = t / R2, jb = t % R2; // layout B: z-lines, row pair (r, r + H), b = jb + R2 k
 
     // everything this thread reads is issued up front: the partial pair, the Boltzmann
@@ -271,13 +286,13 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     double2 z[R1], hz[R1], az[EXTRA ? R1 : 1], sz[EXTRA ? R1 : 1];
 #pragma unroll
     for (int k = 0; k < R1; k++) {
-        const long p = pbase + (long)(j + R2 * k) * Sa + c;
-        z[k] = make_double2(Tin[p], Tin[p + H]);
+        const long p = pbase + (long)(j + R2 * k) * Sa + 2 * c;
+        z[k] = ld2(Tin + p);
         if (finish) {
-            hz[k] = make_double2(hsel[p], hsel[p + H]);
+            hz[k] = ld2(hsel + p);
             if constexpr (EXTRA) {
-                az[k] = make_double2(asel[p], asel[p + H]);
-                sz[k] = make_double2(ssel[p], ssel[p + H]);
+                az[k] = ld2(asel + p);
+                sz[k] = ld2(ssel + p);
             }
         }
     }
@@ -305,7 +320,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
         double* qout = io.qout;
 #pragma unroll
         for (int k = 0; k < R1; k++) {
-            const long p = base + (long)(j + R2 * k) * Sa + c;
+            const long p = base + (long)(j + R2 * k) * Sa + 2 * c;
             if constexpr (MODE == CHAIN_PLAIN) {
                 z[k].x *= hz[k].x;
                 z[k].y *= hz[k].y;
@@ -316,8 +331,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
                 z[k].x = fma(az[k].x, sz[k].x, z[k].x);
                 z[k].y = fma(az[k].y, sz[k].y, z[k].y);
             }
-            qout[p] = z[k].x;
-            qout[p + H] = z[k].y;
+            st2_nt(qout + p, z[k]);
             if constexpr (MODE == CHAIN_ADJOINT) {
                 z[k].x *= hz[k].x;
                 z[k].y *= hz[k].y;
@@ -332,8 +346,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
 #pragma unroll
     for (int k = 0; k < R1; k++) {
         const int ai = j + R2 * k;
-        rp[ai * RP + c] = z[k].x;
-        rp[ai * RP + c + H] = z[k].y;
+        *reinterpret_cast<double2*>(rp + ai * RP + 2 * c) = z[k];
     }
     __syncthreads();
     double2 w[R1];
@@ -347,8 +360,8 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
 #pragma unroll
     for (int k = 0; k < R1; k++) {
         const long p = base + (long)r * Sa + jb + R2 * k;
-        Tout[p] = w[k].x;
-        Tout[p + (long)H * Sa] = w[k].y;
+        st_nt(Tout + p, w[k].x);
+        st_nt(Tout + p + (long)H * Sa, w[k].y);
     }
 }
 
@@ -415,10 +428,23 @@ hipError_t dispatch_edge(int n, int v, const Rep* reps, int nact, int i, int mod
     return hipGetLastError();
 }
 
+// The a-line accesses move two doubles at once: every field must start on 16 bytes.
+bool al16(const void* p) { return (reinterpret_cast<unsigned long long>(p) & 15) == 0; }
+bool io_aligned(const SlabIO& io) {
+    bool ok = al16(io.tin) && al16(io.tout) && al16(io.qout);
+    for (int f = 0; f < 2; f++) ok = ok && al16(io.h[f]) && al16(io.aux[f]) && al16(io.src[f]);
+    return ok;
+}
+
 }  // namespace
 
 hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st, const Rep* one,
                        int variant) {
+    if (one) {  // host copies of the descriptors: check the alignment the kernel relies on
+        for (int r = 0; r < nact; r++)
+            if (!al16(one[r].h) || !al16(one[r].q) || !al16(one[r].T0) || !al16(one[r].T1) || (one[r].M & 1))
+                return hipErrorInvalidValue;
+    }
     if (one && nact == 1) {
         if (i > one->N - 1) return hipSuccess;
         const SlabIO io = plain_io(*one, i);
@@ -428,6 +454,7 @@ hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* t
 }
 
 hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st, int variant) {
+    if (!io_aligned(io) || (io.M & 1)) return hipErrorInvalidValue;
     return dispatch_edge(n, variant, nullptr, 1, 0, mode, &io, tw, st);
 }
 
