@@ -108,6 +108,7 @@ Rep make_rep(const lfts_ctx* c) {
     r.q = c->q;
     r.T0 = c->T0;
     r.T1 = c->T1;
+    r.phacc = c->phacc;
     for (int d = 0; d < 3; d++) r.g[d] = c->ax[d].gnat;
     r.Dh = c->Dh;
     r.wh = c->wh;
@@ -344,10 +345,14 @@ int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep*
 
 // One propagator evaluation (src/diblock.cc:30-60) of every active replica; Q of
 // replica id lands in d_scal[id][0].
-int calc_concs_async(lfts_batch* b) {
+int calc_concs_async(lfts_batch* b, bool store_all) {
     const int nact = (int)b->act.size();
     if (nact == 0) return LFTS_OK;
     if (b->ctx.size() == 1 && b->ctx[0]->ck_stride) return ckpt_calc_concs(b->ctx[0]);
+    // One replica alone runs latency-bound passes: it fuses the density sums into the
+    // second half of its chain (no separate pass over the stored pairs). A set moves the
+    // same bytes either way and keeps the leaner kernel.
+    const int fuse = (!store_all && nact == 1) ? 1 : 0;
     lfts_batch::EvQuad ev{};
     if (b->prof) {
         for (hipEvent_t* e : {&ev.pe0, &ev.ch0, &ev.ch1, &ev.pe1})
@@ -379,7 +384,7 @@ int calc_concs_async(lfts_batch* b) {
         for (int k = 0; k < G.cnt; k++) Ng = std::max(Ng, b->ctx[b->act[G.off + k]]->N);
         const int v = variant[g];
         for (int i = 0; i < Ng; i++) {
-            BHIPCHK(b, lfts::launch_slab(b->d_ract + G.off, G.cnt, G.edge, i, G.tw, s, b->h_ract + G.off, v));
+            BHIPCHK(b, lfts::launch_slab(b->d_ract + G.off, G.cnt, G.edge, i, G.tw, s, b->h_ract + G.off, v, fuse));
             if (i > 0) npasses++;
         }
     }
@@ -400,7 +405,7 @@ int calc_concs_async(lfts_batch* b) {
     if (b->prof) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
     BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_part, b->st));
     BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, 1, 1, b->d_scal, 0, b->st));
-    BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, b->st));
+    BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, fuse, b->st));
     if (b->prof) {
         BHIPCHK(b, hipEventRecord(ev.pe1, b->st));
         ev.npasses = npasses;
@@ -408,8 +413,10 @@ int calc_concs_async(lfts_batch* b) {
         b->ev_pending.push_back(ev);
     }
     for (int id : b->act) {
-        b->ctx[id]->n_pe++;
-        b->ctx[id]->n_cs += b->ctx[id]->N - 1;
+        lfts_ctx* c = b->ctx[id];
+        c->n_pe++;
+        c->n_cs += c->N - 1;
+        c->pairs_all = !lfts::fused_phi(make_rep(c), fuse);
     }
     return LFTS_OK;
 }
@@ -422,7 +429,7 @@ std::vector<int> all_ids(const lfts_batch* b) {
 
 int batch_calc_concs(lfts_batch* b, double* lnQ) {
     if (int rc = set_active(b, all_ids(b))) return rc;
-    if (int rc = calc_concs_async(b)) return rc;
+    if (int rc = calc_concs_async(b, true)) return rc;
     if (int rc = read_scal(b)) return rc;
     bool finite = true;
     for (size_t id = 0; id < b->ctx.size(); id++) {
@@ -735,6 +742,7 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     if (!auto_stride) CREATE_HIP(hipMalloc(&c->q, sizeof(double) * 2 * M * (size_t)(c->N - 1)));
     CREATE_HIP(hipMalloc(&c->T0, sizeof(double) * 2 * M));
     if (c->fused) CREATE_HIP(hipMalloc(&c->T1, sizeof(double) * 2 * M));
+    if (c->fused) CREATE_HIP(hipMalloc(&c->phacc, sizeof(double) * 4 * M));
     CREATE_HIP(hipMalloc(&c->Dh, sizeof(double) * M * (size_t)(c->nhMax + 1)));
     CREATE_HIP(hipMalloc(&c->wh, sizeof(double) * M * (size_t)(c->nhMax + 1)));
     CREATE_HIP(hipMalloc(&c->noise, sizeof(double) * 2 * M));
@@ -771,7 +779,7 @@ int lfts_destroy(lfts_ctx* c) {
         (void)hipFree(p.gnat);
         (void)hipFree(p.kap);
     }
-    for (void* p : {(void*)c->w, (void*)c->h, (void*)c->q, (void*)c->T0, (void*)c->T1, (void*)c->Dh, (void*)c->wh,
+    for (void* p : {(void*)c->w, (void*)c->h, (void*)c->q, (void*)c->T0, (void*)c->T1, (void*)c->phacc, (void*)c->Dh, (void*)c->wh,
                     (void*)c->noise, (void*)c->S, (void*)c->zk, (void*)c->test, (void*)c->dq, (void*)c->dbuf,
                     (void*)c->acc, (void*)c->dred, (void*)c->kry, (void*)c->pmul})
         (void)hipFree(p);
@@ -979,6 +987,10 @@ int lfts_hamiltonian(lfts_ctx* c, double XbN, double* H) {
 int lfts_get_pair(lfts_ctx* c, int i, double* out) {
     if (!c || !out || i < 1 || i > c->N) return fail(c, LFTS_EINVAL, "get_pair: index out of range");
     if (c->ck_stride && i > 1) return fail(c, LFTS_EINVAL, "get_pair: pairs are not stored (checkpointed storage)");
+    if (!c->pairs_all && i > (c->N + 1) / 2 && i != c->N)
+        return fail(c, LFTS_EINVAL,
+                    "get_pair: the last evaluation (an Anderson iteration or the Hamiltonian) kept only pairs "
+                    "1..ceil(N/2) and N; call lfts_calc_concs first");
     HIPCHK(c, hipMemcpyAsync(out, pair_ptr(c, i), sizeof(double) * 2 * c->M, hipMemcpyDeviceToHost, c->self->st));
     HIPCHK(c, hipStreamSynchronize(c->self->st));
     return LFTS_OK;

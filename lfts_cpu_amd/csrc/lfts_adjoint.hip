@@ -349,7 +349,7 @@ namespace {
 int forward(lfts_ctx* c) {
     lfts_batch* b = c->self;
     if (int rc = set_active(b, all_ids(b))) return rc;
-    return calc_concs_async(b);
+    return calc_concs_async(b, true);
 }
 
 const double* Qdev(lfts_ctx* c) { return c->self->d_scal; }

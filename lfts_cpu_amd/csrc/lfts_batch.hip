@@ -85,14 +85,45 @@ __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, co
 
 // src/diblock.cc:51-53,76-95: phiA = sum_{i<=NA} q_i q+_i, phiB = sum_{i>NA} q_i q+_i,
 // normalised by N Q h and written as phi- = phiA - phiB, phi+ = phiA + phiB.
-__global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const double* __restrict__ scal) {
+// The bead sums run in the order of the fused chain (fused_phi): s[0]/s[1] over beads
+// i0+1..N ascending (block A/B), s[2]/s[3] over beads N-i0..1 descending, each started by
+// the first product and continued with fma; the middle bead of an odd N comes last. The
+// stored-pair evaluation repeats that order, so phi is bit-identical either way.
+__global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const double* __restrict__ scal,
+                                             int fuse) {
     REPLICA(R);
     const double Q = scal[(long)R.id * SSTRIDE];
     const long M = R.M;
+    const int N = R.N, NA = R.NA, i0 = (N + 1) / 2;
+    const bool fused = fused_phi(R, fuse);
+    bool u[4];
+    phi_sums_used(N, NA, u);
     FOR_POINTS(r, R) {
-        double pA = 0.0, pB = 0.0;
-        for (int i = 1; i <= R.NA; i++) pA += pair_ptr(R, i)[r] * pair_ptr(R, R.N + 1 - i)[M + r];
-        for (int i = R.NA + 1; i <= R.N; i++) pB += pair_ptr(R, i)[r] * pair_ptr(R, R.N + 1 - i)[M + r];
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        if (fused) {
+            for (int a = 0; a < 4; a++)
+                if (u[a]) s[a] = R.phacc[(long)a * M + r];
+        } else {
+            bool t[4] = {false, false, false, false};
+            for (int j = i0 + 1; j <= N; j++) {
+                const int a = (j <= NA) ? 0 : 1;
+                const double x = rep_pair(R, j)[r], y = rep_pair(R, N + 1 - j)[M + r];  // q_j, q+_j
+                s[a] = t[a] ? fma(x, y, s[a]) : x * y;
+                t[a] = true;
+            }
+            for (int j = N - i0; j >= 1; j--) {
+                const int a = (j <= NA) ? 2 : 3;
+                const double x = rep_pair(R, N + 1 - j)[M + r], y = rep_pair(R, j)[r];  // q+_j, q_j
+                s[a] = t[a] ? fma(x, y, s[a]) : x * y;
+                t[a] = true;
+            }
+        }
+        double pA = s[0] + s[2], pB = s[1] + s[3];
+        if (N & 1) {  // middle bead: both fields of pair i0
+            const double pm = rep_pair(R, i0)[r] * rep_pair(R, i0)[M + r];
+            if (i0 <= NA) pA += pm;
+            else pB += pm;
+        }
         const double phA = pA / (R.N * Q * R.h[r]);
         const double phB = pB / (R.N * Q * R.h[r + M]);
         R.w[2 * M + r] = phA - phB;
@@ -299,8 +330,8 @@ hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncom
                          int scal_off, hipStream_t st) {
     LAUNCH(k_reduce, dim3(nact), reps, part, ncomp, div_by_M, scal, scal_off);
 }
-hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, hipStream_t st) {
-    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal);
+hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, int fuse, hipStream_t st) {
+    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal, fuse);
 }
 hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st) {
     LAUNCH(k_am_dots, dim3(nb_max, nact), reps, coef, part);

@@ -37,8 +37,9 @@
 #include "lfts_internal.h"
 
 // Rounding is fixed by the source, not left to the compiler: fused multiply-adds are
-// written out (fma) where they are wanted and nothing else is contracted, so every
-// kernel shape below computes bit-identical results.
+// written out (fma) where they are wanted and nothing else is contracted (this file is
+// built with -ffp-contract=on, under which the pragma holds), so every kernel below,
+// whatever its shape or the code around the butterflies, computes bit-identical results.
 #pragma clang fp contract(off)
 
 namespace lfts {
@@ -59,12 +60,46 @@ __device__ __forceinline__ double2 c_rot(double2 a) {
     return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
 
+// Workgroup barrier for LDS traffic only. The waves of a slab pass share data through
+// LDS alone (exchange rows, the re-paired plane, the tables), so the barrier waits for
+// LDS operations but not for global loads still in flight (the operands of later steps)
+// or for stores (pairs written while the next convolution runs), as __syncthreads would.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// a*b rounded once, as an fma with a +0 addend: the compiler may not merge it into a
+// following add (it may for a plain product, and whether it does depends on the code
+// around it), so every kernel shape continues the chain from exactly the stored value.
+__device__ __forceinline__ double rmul(double a, double b) { return fma(a, b, 0.0); }
+
+// Every global access of the slab pass names the global address space. Its pointers may
+// come from descriptors read from memory, where the compiler cannot tell and would emit
+// flat accesses: slower, and counted with the LDS operations, so that an LDS barrier
+// would wait for them.
 typedef double d2v __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
-__device__ __forceinline__ void st_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
+typedef __attribute__((address_space(1))) const d2v gcd2v;
+typedef __attribute__((address_space(1))) d2v gd2v;
+typedef __attribute__((address_space(1))) const double gcd;
+typedef __attribute__((address_space(1))) double gd;
+__device__ __forceinline__ double2 ld2(const double* p) {
+    const d2v w = *(gcd2v*)(p);
+    return make_double2(w.x, w.y);
+}
+__device__ __forceinline__ double2 ld2(const double2* p) { return ld2(reinterpret_cast<const double*>(p)); }
+__device__ __forceinline__ double ldg(const double* p) { return *(gcd*)(p); }
+// non-temporal: data read once and not kept in L2 (a cached line the pass then
+// overwrites would be left dirty, to be written back when the kernel ends)
+__device__ __forceinline__ double2 ld2_nt(const double* p) {
+    const d2v w = __builtin_nontemporal_load((gcd2v*)(p));
+    return make_double2(w.x, w.y);
+}
+__device__ __forceinline__ void st_nt(double* p, double v) { __builtin_nontemporal_store(v, (gd*)(p)); }
 __device__ __forceinline__ void st2_nt(double* p, double2 v) {
     const d2v w = {v.x, v.y};
-    __builtin_nontemporal_store(w, reinterpret_cast<d2v*>(p));
+    __builtin_nontemporal_store(w, (gd2v*)(p));
 }
 
 template <bool INV>
@@ -85,34 +120,51 @@ __device__ __forceinline__ void dft4(double2& v0, double2& v1, double2& v2, doub
     v3 = c_sub(d02, d13);
 }
 
+// radix-8 DFT after its first radix-2 stage (a_n = v_n + v_{n+4}, b_n = v_n - v_{n+4}):
+// twiddles w8^n on b_n and two radix-4 DFTs, outputs in natural order. The w8 and w8^3
+// twiddles share the factor r = 1/sqrt2: with b1 w8 = r P and b3 w8^3 = r Q, the b-DFT
+// needs only r (P + Q) and r rot(P - Q), each folded into its final add as an fma.
+template <bool INV>
+__device__ __forceinline__ void dft8_tail(double2* a, double2* b, double2* v) {
+    const double r = 0.70710678118654752440;
+    // w8 = (1 - i)/sqrt2 forward, (1 + i)/sqrt2 inverse; w8^3 = (-1 - i)/sqrt2, (-1 + i)/sqrt2
+    const double2 P = INV ? make_double2(b[1].x - b[1].y, b[1].x + b[1].y) : make_double2(b[1].x + b[1].y, b[1].y - b[1].x);
+    const double2 Q = INV ? make_double2(-(b[3].x + b[3].y), b[3].x - b[3].y) : make_double2(b[3].y - b[3].x, -(b[3].x + b[3].y));
+    const double2 S = c_add(P, Q), D = c_rot<INV>(c_sub(P, Q));
+    const double2 b2 = c_rot<INV>(b[2]);
+    const double2 s02 = c_add(b[0], b2), d02 = c_sub(b[0], b2);
+    dft4<INV>(a[0], a[1], a[2], a[3]);
+    v[0] = a[0]; v[2] = a[1]; v[4] = a[2]; v[6] = a[3];
+    v[1] = make_double2(fma(r, S.x, s02.x), fma(r, S.y, s02.y));
+    v[5] = make_double2(fma(-r, S.x, s02.x), fma(-r, S.y, s02.y));
+    v[3] = make_double2(fma(r, D.x, d02.x), fma(r, D.y, d02.y));
+    v[7] = make_double2(fma(-r, D.x, d02.x), fma(-r, D.y, d02.y));
+}
+
 // natural-order radix-8 DFT: radix-2 split, twiddles w8^n, two radix-4 DFTs
 template <bool INV>
 __device__ __forceinline__ void dft8(double2* v) {
-    const double r = 0.70710678118654752440;
-    double2 a0 = c_add(v[0], v[4]), b0 = c_sub(v[0], v[4]);
-    double2 a1 = c_add(v[1], v[5]), b1 = c_sub(v[1], v[5]);
-    double2 a2 = c_add(v[2], v[6]), b2 = c_sub(v[2], v[6]);
-    double2 a3 = c_add(v[3], v[7]), b3 = c_sub(v[3], v[7]);
-    // b_n *= w8^n; w8 = (1 - i)/sqrt2 forward, (1 + i)/sqrt2 inverse
-    b1 = INV ? make_double2(r * (b1.x - b1.y), r * (b1.x + b1.y)) : make_double2(r * (b1.x + b1.y), r * (b1.y - b1.x));
-    b2 = c_rot<INV>(b2);
-    b3 = INV ? make_double2(-r * (b3.x + b3.y), r * (b3.x - b3.y)) : make_double2(r * (b3.y - b3.x), -r * (b3.x + b3.y));
-    dft4<INV>(a0, a1, a2, a3);
-    dft4<INV>(b0, b1, b2, b3);
-    v[0] = a0; v[2] = a1; v[4] = a2; v[6] = a3;
-    v[1] = b0; v[3] = b1; v[5] = b2; v[7] = b3;
+    double2 a[4], b[4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        a[n] = c_add(v[n], v[n + 4]);
+        b[n] = c_sub(v[n], v[n + 4]);
+    }
+    dft8_tail<INV>(a, b, v);
 }
 
-// natural-order radix-3 DFT
+// natural-order radix-3 DFT; the 1/2 and sqrt(3)/2 products are folded into their adds
 template <bool INV>
 __device__ __forceinline__ void dft3(double2& a, double2& b, double2& c) {
     const double s3 = 0.86602540378443864676;  // sqrt(3)/2
     const double2 t = c_add(b, c), d = c_sub(b, c);
-    const double2 m = make_double2(a.x - 0.5 * t.x, a.y - 0.5 * t.y);
-    const double2 isd = make_double2(-s3 * d.y, s3 * d.x);  // i (sqrt3/2) d
+    const double2 m = make_double2(fma(-0.5, t.x, a.x), fma(-0.5, t.y, a.y));
     a = c_add(a, t);
-    b = INV ? c_add(m, isd) : c_sub(m, isd);
-    c = INV ? c_sub(m, isd) : c_add(m, isd);
+    // i (sqrt3/2) d = (-s3 d.y, s3 d.x); forward: b = m - i s3 d, c = m + i s3 d
+    const double2 mi = make_double2(fma(s3, d.y, m.x), fma(-s3, d.x, m.y));  // m - i s3 d
+    const double2 pi = make_double2(fma(-s3, d.y, m.x), fma(s3, d.x, m.y));  // m + i s3 d
+    b = INV ? pi : mi;
+    c = INV ? mi : pi;
 }
 
 // natural-order radix-12 DFT as 4 x 3: x[n2 + 3 n1] -> radix-4 over n1, twiddle
@@ -148,6 +200,31 @@ __device__ __forceinline__ void dft12(double2* v) {
     }
 }
 
+// Inverse DFT of length R of v_n = y_n g_n (g real): the products are folded into the
+// first butterfly stage (pairs n, n + R/2) as fma, then the DFT continues as dft<R, true>.
+template <int R>
+__device__ __forceinline__ void inv_dft_g(double2* y, const double* g) {
+    static_assert(R == 4 || R == 8, "kernel-multiply fusion for radix 4 and 8");
+    constexpr int Hh = R / 2;
+    double2 a[Hh], b[Hh];
+#pragma unroll
+    for (int n = 0; n < Hh; n++) {
+        const double2 t = make_double2(y[n + Hh].x * g[n + Hh], y[n + Hh].y * g[n + Hh]);
+        a[n] = make_double2(fma(y[n].x, g[n], t.x), fma(y[n].y, g[n], t.y));
+        b[n] = make_double2(fma(y[n].x, g[n], -t.x), fma(y[n].y, g[n], -t.y));
+    }
+    if constexpr (R == 8) {
+        dft8_tail<true>(a, b, y);
+    } else {
+        // radix 4: a = (s02, s13), b = (d02, v1 - v3)
+        const double2 d13 = c_rot<true>(b[1]);
+        y[0] = c_add(a[0], a[1]);
+        y[2] = c_sub(a[0], a[1]);
+        y[1] = c_add(b[0], d13);
+        y[3] = c_sub(b[0], d13);
+    }
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void dft(double2* v) {
     if constexpr (R == 12) dft12<INV>(v);
@@ -167,11 +244,11 @@ struct RegTab {
     __device__ __forceinline__ void load(const double2* __restrict__ twt, const double* __restrict__ gt, int j) {
         constexpr int U = R1 / R2;
 #pragma unroll
-        for (int k = 0; k < R1; k++) tw[k] = twt[j * k];
+        for (int k = 0; k < R1; k++) tw[k] = ld2(twt + j * k);
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int f = 0; f < R2; f++) g[u * R2 + f] = gt[j * U + u + R1 * f];
+            for (int f = 0; f < R2; f++) g[u * R2 + f] = ldg(gt + j * U + u + R1 * f);
     }
     __device__ __forceinline__ double2 w(int k) const { return tw[k]; }
     __device__ __forceinline__ double gv(int u, int f) const { return g[u * R2 + f]; }
@@ -201,10 +278,10 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
     dft<R1, false>(z);
 #pragma unroll
     for (int k = 1; k < R1; k++) z[k] = c_mul(z[k], tab.w(k));
-    __syncthreads();  // previous readers of the exchange rows are done
+    lds_barrier();  // previous readers of the exchange rows are done
 #pragma unroll
     for (int k = 0; k < R1; k++) lx[k * Q + j] = z[k];
-    __syncthreads();
+    lds_barrier();
     double2 y[R1];
 #pragma unroll
     for (int u = 0; u < U; u++)
@@ -213,20 +290,17 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 #pragma unroll
     for (int u = 0; u < U; u++) {
         dft<R2, false>(y + u * R2);
+        double gl[R2];
 #pragma unroll
-        for (int f = 0; f < R2; f++) {
-            const double gg = tab.gv(u, f);
-            y[u * R2 + f].x *= gg;
-            y[u * R2 + f].y *= gg;
-        }
-        dft<R2, true>(y + u * R2);
+        for (int f = 0; f < R2; f++) gl[f] = tab.gv(u, f);
+        inv_dft_g<R2>(y + u * R2, gl);
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
         for (int jj = 0; jj < R2; jj++) lx[(j * U + u) * Q + jj] = y[u * R2 + jj];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < R1; k++) z[k] = lx[k * Q + j];
 #pragma unroll
@@ -244,7 +318,18 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 //   CHAIN_TANGENT  out = h u + aux src         (directional derivative of the chain)
 //   CHAIN_ADJOINT  out = u + aux src, stored; the chain continues with h out
 // Every variant performs the same arithmetic on every element: results are bit-identical.
-template <int E, int R1, int R2, int MODE, int TWL, int FPW>
+// FUSE = 1 (plain chain, io.acc set): the pass also adds its new values times the
+// partner values to the density sums (acc = fma(new, partner, acc), or the product on
+// first touch). Only the one-replica kernel uses it: it runs one wave per SIMD, so the
+// extra live registers cost nothing, while the replica-set kernel needs them for
+// occupancy (and gains nothing there: the bytes moved are the same either way).
+// The loads are unconditional (a pass without a finish reads its own input in place of
+// h and the derivative factors), so the compiler can count them exactly and the first
+// convolution waits only for what it reads.
+// FIN = 0 only for a chain's first pass (no step to finish): fixed per kernel, so the
+// code is straight and the compiler's wait counts stay exact (a run-time branch around
+// the finish would make it wait for every store before the next convolution).
+template <int E, int R1, int R2, int MODE, int TWL, int FPW, int FUSE, int FIN>
 __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const double2* __restrict__ tw) {
     static_assert(R1 * R2 == E && R1 % R2 == 0, "unsupported factorisation");
     constexpr int H = E / 2;           // complex lines per plane
@@ -256,11 +341,13 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
     constexpr int TBYTES = TWL ? E * 16 + 2 * E * 8 : 0;
     constexpr bool EXTRA = MODE != CHAIN_PLAIN;
+    constexpr bool PLAIN = MODE == CHAIN_PLAIN && FUSE;
     __shared__ __align__(16) unsigned char smem[FPW * BYTES + TBYTES];
     if (io.skip && *io.skip) return;
 
     const bool xslab = io.xslab != 0;
-    const bool finish = io.finish != 0, cont = io.cont != 0;
+    constexpr bool finish = FIN != 0;
+    const bool cont = io.cont != 0;
     const long M = io.M;
     const long plane_stride = xslab ? (long)E * E : E;
     const long Sa = xslab ? E : (long)E * E;
@@ -277,53 +364,89 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     const int c = t % H, j = t / H;    // layout A: a-lines, column pair (2c, 2c + 1), a = j + R2 k
     const int r = t / R2, jb = t % R2; // layout B: z-lines, row pair (r, r + H), b = jb + R2 k
 
-    // everything this thread reads is issued up front: the partial pair, the Boltzmann
-    // factor of its field at this bead (src/step.cc:53-64), twiddles and kernel values
+    // everything this thread reads is issued up front, in the order it is needed (loads
+    // complete in order): the partial pair, the twiddle and kernel tables, then for the
+    // finish the Boltzmann factor of its field at this bead (src/step.cc:53-64), the
+    // derivative factors and the density-sum operands
     const double* Tin = io.tin + (long)f * M;
-    const double* hsel = f ? io.h[1] : io.h[0];
-    const double* asel = f ? io.aux[1] : io.aux[0];
-    const double* ssel = f ? io.src[1] : io.src[0];
+    const double* hsel = finish ? (f ? io.h[1] : io.h[0]) : Tin;
+    const double* asel = finish ? (f ? io.aux[1] : io.aux[0]) : Tin;
+    const double* ssel = finish ? (f ? io.src[1] : io.src[0]) : Tin;
     double2 z[R1], hz[R1], az[EXTRA ? R1 : 1], sz[EXTRA ? R1 : 1];
-#pragma unroll
-    for (int k = 0; k < R1; k++) {
-        const long p = pbase + (long)(j + R2 * k) * Sa + 2 * c;
-        z[k] = ld2(Tin + p);
-        if (finish) {
-            hz[k] = ld2(hsel + p);
-            if constexpr (EXTRA) {
-                az[k] = ld2(asel + p);
-                sz[k] = ld2(ssel + p);
-            }
-        }
+    // fused density sums: partner values and running sums of this field (a pass that
+    // fuses always finishes a step)
+    const double* psel = nullptr;
+    double* accp = nullptr;
+    bool first = false;
+    if constexpr (PLAIN) {
+        psel = f ? io.part[1] : io.part[0];
+        accp = f ? io.acc[1] : io.acc[0];
+        first = f ? io.first[1] != 0 : io.first[0] != 0;
     }
+    double2 pz[PLAIN ? R1 : 1], oz[PLAIN ? R1 : 1];
+#pragma unroll
+    for (int k = 0; k < R1; k++) z[k] = ld2(Tin + pbase + (long)(j + R2 * k) * Sa + 2 * c);
     using Tab = typename std::conditional<TWL != 0, LdsTab<R1, R2>, RegTab<R1, R2>>::type;
     Tab tabA, tabB;
+    // LDS tables: one entry per thread (the workgroup has at least E threads)
+    const bool tl = TWL != 0 && threadIdx.x < E;
+    double2 tv{};
+    double gav = 0.0, gbv = 0.0;
     if constexpr (TWL != 0) {
-        double2* stw = reinterpret_cast<double2*>(smem + FPW * BYTES);
-        double* sga = reinterpret_cast<double*>(smem + FPW * BYTES + E * 16);
-        double* sgb = sga + E;
-        for (int k = threadIdx.x; k < E; k += blockDim.x) {
-            stw[k] = tw[k];
-            sga[k] = ga[k];
-            sgb[k] = gb[k];
+        if (tl) {
+            tv = ld2(tw + threadIdx.x);
+            gav = ldg(ga + threadIdx.x);
+            gbv = ldg(gb + threadIdx.x);
         }
-        __syncthreads();
-        tabA.load(stw, sga, j);
-        tabB.load(stw, sgb, jb);
     } else {
         tabA.load(tw, ga, j);
         tabB.load(tw, gb, jb);
     }
+#pragma unroll
+    for (int k = 0; k < R1; k++) {
+        const long p = pbase + (long)(j + R2 * k) * Sa + 2 * c;
+        hz[k] = ld2(hsel + p);
+        if constexpr (EXTRA) {
+            az[k] = ld2(asel + p);
+            sz[k] = ld2(ssel + p);
+        }
+    }
+    if constexpr (PLAIN) {
+#pragma unroll
+        for (int k = 0; k < R1; k++) {
+            const long p = pbase + (long)(j + R2 * k) * Sa + 2 * c;
+            pz[k] = ld2_nt(psel + p);
+            oz[k] = ld2_nt(accp + p);  // unused on first touch
+        }
+    }
+    if constexpr (TWL != 0) {
+        double2* stw = reinterpret_cast<double2*>(smem + FPW * BYTES);
+        double* sga = reinterpret_cast<double*>(smem + FPW * BYTES + E * 16);
+        double* sgb = sga + E;
+        if (tl) {
+            stw[threadIdx.x] = tv;
+            sga[threadIdx.x] = gav;
+            sgb[threadIdx.x] = gbv;
+        }
+        lds_barrier();
+        tabA.load(stw, sga, j);
+        tabB.load(stw, sgb, jb);
+    }
 
-    if (finish) {
+    if constexpr (finish) {
         conv_line<R1, R2>(z, xbuf + c * XP, j, tabA);
         double* qout = io.qout;
 #pragma unroll
         for (int k = 0; k < R1; k++) {
             const long p = base + (long)(j + R2 * k) * Sa + 2 * c;
             if constexpr (MODE == CHAIN_PLAIN) {
-                z[k].x *= hz[k].x;
-                z[k].y *= hz[k].y;
+                z[k].x = rmul(z[k].x, hz[k].x);
+                z[k].y = rmul(z[k].y, hz[k].y);
+                if constexpr (PLAIN) {
+                    const double ax = z[k].x * pz[k].x, ay = z[k].y * pz[k].y;
+                    const double bx = fma(z[k].x, pz[k].x, oz[k].x), by = fma(z[k].y, pz[k].y, oz[k].y);
+                    st2_nt(accp + (p - (long)f * M), first ? make_double2(ax, ay) : make_double2(bx, by));
+                }
             } else if constexpr (MODE == CHAIN_TANGENT) {
                 z[k].x = fma(z[k].x, hz[k].x, az[k].x * sz[k].x);
                 z[k].y = fma(z[k].y, hz[k].y, az[k].y * sz[k].y);
@@ -331,7 +454,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
                 z[k].x = fma(az[k].x, sz[k].x, z[k].x);
                 z[k].y = fma(az[k].y, sz[k].y, z[k].y);
             }
-            st2_nt(qout + p, z[k]);
+            if (!PLAIN || qout) st2_nt(qout + p, z[k]);
             if constexpr (MODE == CHAIN_ADJOINT) {
                 z[k].x *= hz[k].x;
                 z[k].y *= hz[k].y;
@@ -342,13 +465,13 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
 
     conv_line<R1, R2>(z, xbuf + c * XP, j, tabA);
     // re-pair the real plane: rows (r, r + H) become the complex z-lines
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < R1; k++) {
         const int ai = j + R2 * k;
         *reinterpret_cast<double2*>(rp + ai * RP + 2 * c) = z[k];
     }
-    __syncthreads();
+    lds_barrier();
     double2 w[R1];
 #pragma unroll
     for (int k = 0; k < R1; k++) {
@@ -367,21 +490,24 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
 
 // Plain chains of a replica set. Workgroup (slot * E + plane, field): slot picks the
 // replica descriptor.
-template <int E, int R1, int R2, int TWL, int FPW>
+template <int E, int R1, int R2, int TWL, int FPW, int FIN>
 __global__ __launch_bounds__((E / 2) * R2 * FPW) void k_slab(const Rep* __restrict__ reps,
                                                            const double2* __restrict__ tw, int i) {
     const int slot = blockIdx.x / E, plane = blockIdx.x - slot * E;
     const Rep& R = reps[slot];  // fields read in place (a private copy would live in scratch)
     if (i > R.N - 1) return;    // this replica's chain is complete
     const SlabIO io = plain_io(R, i);
-    slab_pass<E, R1, R2, CHAIN_PLAIN, TWL, FPW>(io, plane, tw);
+    slab_pass<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, FIN>(io, plane, tw);
 }
 
 // One replica, its pass addresses in the kernel arguments (no dependent descriptor load
 // before the first field loads).
-template <int E, int R1, int R2, int MODE, int TWL, int FPW>
-__global__ __launch_bounds__((E / 2) * R2 * FPW) void k_slab_io(const SlabIO io, const double2* __restrict__ tw) {
-    slab_pass<E, R1, R2, MODE, TWL, FPW>(io, blockIdx.x, tw);
+// One replica's pass has E * 2 / FPW workgroups, at most one per CU: the compiler may
+// schedule for one wave per SIMD (registers are not the limit there).
+template <int E, int R1, int R2, int MODE, int TWL, int FPW, int FUSE, int FIN>
+__global__ __launch_bounds__((E / 2) * R2 * FPW) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_slab_io(const SlabIO io, const double2* __restrict__ tw) {
+    slab_pass<E, R1, R2, MODE, TWL, FPW, FUSE, FIN>(io, blockIdx.x, tw);
 }
 
 bool slab_supported(int n0, int n1, int n2) {
@@ -393,18 +519,28 @@ namespace {
 template <int E, int R1, int R2, int TWL, int FPW>
 void launch_set(const Rep* reps, int nact, int i, const double2* tw, hipStream_t st) {
     const dim3 grid(nact * E, 2 / FPW), block((E / 2) * R2 * FPW);
-    hipLaunchKernelGGL((k_slab<E, R1, R2, TWL, FPW>), grid, block, 0, st, reps, tw, i);
+    if (i >= 1) hipLaunchKernelGGL((k_slab<E, R1, R2, TWL, FPW, 1>), grid, block, 0, st, reps, tw, i);
+    else hipLaunchKernelGGL((k_slab<E, R1, R2, TWL, FPW, 0>), grid, block, 0, st, reps, tw, i);
 }
 
 template <int E, int R1, int R2, int TWL, int FPW>
 void launch_one(int mode, const SlabIO& io, const double2* tw, hipStream_t st) {
     const dim3 grid(E, 2 / FPW), block((E / 2) * R2 * FPW);
-    if (mode == CHAIN_PLAIN)
-        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN, TWL, FPW>), grid, block, 0, st, io, tw);
+    const bool fin = io.finish != 0;
+    if (mode == CHAIN_PLAIN && io.acc[0] && fin)
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 1, 1>), grid, block, 0, st, io, tw);
+    else if (mode == CHAIN_PLAIN && fin)
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, 1>), grid, block, 0, st, io, tw);
+    else if (mode == CHAIN_PLAIN)
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, 0>), grid, block, 0, st, io, tw);
+    else if (mode == CHAIN_TANGENT && fin)
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_TANGENT, TWL, FPW, 0, 1>), grid, block, 0, st, io, tw);
     else if (mode == CHAIN_TANGENT)
-        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_TANGENT, TWL, FPW>), grid, block, 0, st, io, tw);
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_TANGENT, TWL, FPW, 0, 0>), grid, block, 0, st, io, tw);
+    else if (fin)
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_ADJOINT, TWL, FPW, 0, 1>), grid, block, 0, st, io, tw);
     else
-        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_ADJOINT, TWL, FPW>), grid, block, 0, st, io, tw);
+        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_ADJOINT, TWL, FPW, 0, 0>), grid, block, 0, st, io, tw);
 }
 
 // variant v: bit 0 = tables in LDS, bit 1 = both fields in one workgroup
@@ -432,24 +568,27 @@ hipError_t dispatch_edge(int n, int v, const Rep* reps, int nact, int i, int mod
 bool al16(const void* p) { return (reinterpret_cast<unsigned long long>(p) & 15) == 0; }
 bool io_aligned(const SlabIO& io) {
     bool ok = al16(io.tin) && al16(io.tout) && al16(io.qout);
-    for (int f = 0; f < 2; f++) ok = ok && al16(io.h[f]) && al16(io.aux[f]) && al16(io.src[f]);
+    for (int f = 0; f < 2; f++)
+        ok = ok && al16(io.h[f]) && al16(io.aux[f]) && al16(io.src[f]) && al16(io.part[f]) && al16(io.acc[f]);
     return ok;
 }
 
 }  // namespace
 
 hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st, const Rep* one,
-                       int variant) {
+                       int variant, int fuse) {
     if (one) {  // host copies of the descriptors: check the alignment the kernel relies on
         for (int r = 0; r < nact; r++)
-            if (!al16(one[r].h) || !al16(one[r].q) || !al16(one[r].T0) || !al16(one[r].T1) || (one[r].M & 1))
+            if (!al16(one[r].h) || !al16(one[r].q) || !al16(one[r].T0) || !al16(one[r].T1) ||
+                !al16(one[r].phacc) || (one[r].M & 1))
                 return hipErrorInvalidValue;
     }
     if (one && nact == 1) {
         if (i > one->N - 1) return hipSuccess;
-        const SlabIO io = plain_io(*one, i);
+        const SlabIO io = plain_io(*one, i, fuse);
         return dispatch_edge(n, variant, nullptr, 1, i, CHAIN_PLAIN, &io, tw, st);
     }
+    if (fuse) return hipErrorInvalidValue;  // fused sums run only in the one-replica kernel
     return dispatch_edge(n, variant, reps, nact, i, CHAIN_PLAIN, nullptr, tw, st);
 }
 

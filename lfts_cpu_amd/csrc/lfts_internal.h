@@ -44,6 +44,7 @@ struct Rep {
     int id;             // index of the replica in its set (scalar results, coefficients)
     int nb;             // workgroups of the element-wise passes, blocks_for(M)
     const int* skip;    // device flag: when set, this replica's kernels return at once (or null)
+    double* phacc;      // 4M bead sums of the fused evaluation (slab path), or null
 };
 
 // Per-replica coefficients of one Anderson update or Langevin step.
@@ -109,13 +110,38 @@ struct SlabIO {
     const double* src[2];  // the stored values aux multiplies, per field
     const double* g[3];    // per-axis kernels, natural frequency order
     const int* skip;       // device flag: when set, the pass returns at once (or null)
+    // plain chain, fused density sums: the field's new values times the partner values
+    // (the other field of an earlier pair) are added to acc (first: acc = product)
+    const double* part[2];
+    double* acc[2];
+    int first[2];
     long M;
     int finish, cont, xslab;
 };
 
+__host__ __device__ inline const double* rep_pair(const Rep& R, int j) {
+    return (j == 1) ? R.h : R.q + 2L * (j - 2) * R.M;
+}
+
+// Fused density sums (slab path, N >= 3). Pass i >= i0 = ceil(N/2) produces q_{i+1} and
+// q+_{N-i}; their partners q+_{i+1} and q_{N-i} are the two fields of pair N - i, made by
+// an earlier pass. So these passes add the bead products to four sums (field 0 beads
+// i0+1..N ascending, field 1 beads N-i0..1 descending, each split by block A/B) instead
+// of storing their pairs; only pairs 1..i0 and N are stored. For odd N the middle bead
+// i0 pairs two fields of one pair and is added at the end.
+__host__ __device__ inline bool fused_phi(const Rep& R, int fuse) { return fuse && R.phacc && R.n && R.N >= 3; }
+// which of the four sums (field 0 A, field 0 B, field 1 A, field 1 B) receive a bead
+__host__ __device__ inline void phi_sums_used(int N, int NA, bool u[4]) {
+    const int i0 = (N + 1) / 2;
+    u[0] = i0 + 1 <= NA;
+    u[1] = NA < N;
+    u[2] = NA >= 1 && N - i0 >= 1;
+    u[3] = N - i0 > NA;
+}
+
 // Pass i of a replica's plain chain: pair 1 = h, pair i+1 = h_sel C pair_i
 // (src/diblock.cc:40-43, src/step.cc:53-64); T0/T1 alternate as partial pairs.
-__host__ __device__ inline SlabIO plain_io(const Rep& R, int i) {
+__host__ __device__ inline SlabIO plain_io(const Rep& R, int i, int fuse = 0) {
     SlabIO io{};
     const long M = R.M;
     io.tin = (i == 0) ? R.h : ((i & 1) ? R.T0 : R.T1);
@@ -129,6 +155,21 @@ __host__ __device__ inline SlabIO plain_io(const Rep& R, int i) {
     io.finish = i >= 1;
     io.cont = i < R.N - 1;
     io.xslab = (i % 2 == 0);
+    const int i0 = (R.N + 1) / 2;
+    if (fused_phi(R, fuse) && i >= i0) {
+        const double* P = rep_pair(R, R.N - i);
+        const int j0 = i + 1, j1 = R.N - i;  // beads of the two new fields
+        const int b0 = (j0 <= R.NA) ? 0 : 1, b1 = (j1 <= R.NA) ? 0 : 1;
+        io.part[0] = P + M;  // q+_{j0}
+        io.part[1] = P;      // q_{j1}
+        io.acc[0] = R.phacc + (long)b0 * M;
+        io.acc[1] = R.phacc + (long)(2 + b1) * M;
+        const int lo0 = (i0 + 1 > R.NA + 1) ? i0 + 1 : R.NA + 1;
+        const int hi1 = (R.N - i0 < R.NA) ? R.N - i0 : R.NA;
+        io.first[0] = (b0 == 0) ? (j0 == i0 + 1) : (j0 == lo0);
+        io.first[1] = (b1 == 1) ? (j1 == R.N - i0) : (j1 == hi1);
+        if (i + 1 != R.N) io.qout = nullptr;  // pair N is kept for Q
+    }
     return io;
 }
 // Step i = 0 is the prologue (x-slab pass on q_1); step i >= 1 finishes contour step i
@@ -141,8 +182,10 @@ bool slab_supported(int n0, int n1, int n2);
 // give bit-identical results; the fastest one per mesh size and set size is picked by
 // timing (the autotuner of the replica sets).
 constexpr int SLAB_VARIANTS = 4;
+// fuse = 1: density sums fused into the second half of the chain (fused_phi); only for
+// one replica (nact == 1 with its host descriptor `one`).
 hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st,
-                       const Rep* one = nullptr, int variant = 0);
+                       const Rep* one = nullptr, int variant = 0, int fuse = 0);
 // One pass of one replica's chain of kind `mode` (CHAIN_*), over the n planes.
 hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st, int variant = 0);
 
@@ -153,7 +196,9 @@ hipError_t launch_prepare_h(const Rep* reps, int nact, int nb_max, hipStream_t s
 hipError_t launch_q_sum(const Rep* reps, int nact, int nb_max, double* part, hipStream_t st);
 hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncomp, int div_by_M, double* scal,
                          int scal_off, hipStream_t st);
-hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, hipStream_t st);
+// phi from the fused sums (replicas with fused_phi(R, fuse)) or from the stored pairs,
+// both in the same order of operations
+hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, int fuse, hipStream_t st);
 hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st);
 hipError_t launch_am_update(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st);
 hipError_t launch_langevin(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st);

@@ -39,6 +39,8 @@ struct lfts_ctx {
     double* q = nullptr;      // (N-1) pairs
     double* T0 = nullptr;     // 2M partial pair (slab path) / line-path scratch
     double* T1 = nullptr;     // 2M partial pair (slab path)
+    double* phacc = nullptr;  // 4M fused density sums (slab path)
+    bool pairs_all = true;    // the last evaluation stored every pair
     double* Dh = nullptr;     // (nhMax+1) M
     double* wh = nullptr;     // (nhMax+1) M
     double* noise = nullptr;  // 2M: eta_prev | staging for a host eta_new
@@ -139,7 +141,9 @@ double* pair_ptr(lfts_ctx* c, int i);
 int line_contour_step(lfts_ctx* c, const double* in, double* out, const double* h, int i, double* tmp,
                       hipStream_t st);
 int set_active(lfts_batch* b, const std::vector<int>& ids);
-int calc_concs_async(lfts_batch* b);
+// store_all = false lets slab-path replicas fuse the density sums into the chain, which
+// keeps only pairs 1..ceil(N/2) and N; store_all = true keeps every pair.
+int calc_concs_async(lfts_batch* b, bool store_all = false);
 int slab_variant(lfts_batch* b, int edge, int cnt, const lfts::Rep* d_reps, const lfts::Rep* h_reps,
                  const double2* tw, hipStream_t s, int* variant);
 int read_scal(lfts_batch* b);

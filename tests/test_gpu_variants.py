@@ -1,6 +1,7 @@
 """The contour-step kernel shapes the autotuner chooses between (twiddle/kernel tables in
 registers or LDS; one or two fields per workgroup) compute bit-identical results, for
-the plain chain alone and in a replica set, and for the tangent and adjoint chains."""
+the plain chain alone (with and without the fused density sums) and in a replica set,
+and for the tangent and adjoint chains."""
 import numpy as np
 import pytest
 
@@ -30,11 +31,13 @@ def test_every_shape_gives_the_same_bits(gpu, monkeypatch, n):
         lnq = c.calc_concs()
         assert f"variant {variant} (fixed)" in c.tuning()
         got = (lnq, c.get_fields(4), c.pair(N), c.jvp(None, v), c.vjp(ub))
+        H = c.hamiltonian(17.0)  # density sums fused into the chain
+        got = got + (c.get_fields(4),)
         c.close()
         if ref is None:
-            ref = got
+            ref, H_ref = got, H
         else:
-            assert got[0] == ref[0]
+            assert got[0] == ref[0] and H == H_ref
             for a, b in zip(got[1:], ref[1:]):
                 assert np.array_equal(a, b)
 
