@@ -413,8 +413,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": measured_traffic(args.mesh),
-            "kernel": "contour step (48*M algorithmic bytes), HIP events on the context stream over "
-                      f"{PROF_STEPS} further steps",
+            "kernel": "contour step (48*M algorithmic bytes; plain and fused-density-sum passes, 44:45 per "
+                      f"evaluation), HIP events on the context stream over {PROF_STEPS} further steps",
             "contour_step_us": cs_ms * 1e3,
             "slab_tuning": tuning_plain,
         },
