@@ -249,7 +249,7 @@ int set_active(lfts_batch* b, const std::vector<int>& ids) {
     b->act.clear();
     b->groups.clear();
     b->line_reps.clear();
-    for (int edge : {64, 48, 32}) {
+    for (int edge : lfts::SLAB_EDGES) {
         const int off = (int)b->act.size();
         for (int id : ids)
             if (b->ctx[id]->fused && b->ctx[id]->d.m[0] == edge) b->act.push_back(id);
@@ -264,6 +264,8 @@ int set_active(lfts_batch* b, const std::vector<int>& ids) {
             b->line_reps.push_back(id);
             b->act.push_back(id);
         }
+    if (b->act.size() != ids.size())  // a slab-path replica of an edge with no launch group
+        return bfail(b, LFTS_EINVAL, "set_active: a replica's mesh has no slab launch group");
     if (b->act.empty()) return LFTS_OK;
     b->nb_max = 1;
     for (size_t s = 0; s < b->act.size(); s++) {
@@ -306,7 +308,10 @@ int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep*
     int best = 0;
     const char* env = std::getenv("LFTS_SLAB_VARIANT");
     char line[160];
-    if (env && *env >= '0' && *env < '0' + lfts::SLAB_VARIANTS && env[1] == 0) {
+    if (edge == 128) {  // one shape exists at this size (LDS tables, one field per workgroup)
+        best = 1;
+        std::snprintf(line, sizeof line, "edge %d x %d: variant 1 (the only shape)\n", edge, cls);
+    } else if (env && *env >= '0' && *env < '0' + lfts::SLAB_VARIANTS && env[1] == 0) {
         best = *env - '0';
         std::snprintf(line, sizeof line, "edge %d x %d: variant %d (fixed)\n", edge, cls, best);
     } else {
@@ -367,7 +372,7 @@ int calc_concs_async(lfts_batch* b, bool store_all) {
     const int ng = (int)b->groups.size();
     const bool fork = ng > 1;
     // kernel shapes, tuned (on first use of a size class) before the groups run concurrently
-    int variant[3] = {0, 0, 0};
+    int variant[lfts::N_SLAB_EDGES] = {};
     for (int g = 0; g < ng; g++) {
         const auto& G = b->groups[g];
         if (int rc = slab_variant(b, G.edge, G.cnt, b->d_ract + G.off, b->h_ract + G.off, G.tw, b->st, &variant[g]))
@@ -662,7 +667,7 @@ int batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
     } while (0)
     BCREATE(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
     if (n > 1) {
-        for (int g = 0; g < 3; g++) {
+        for (int g = 0; g < lfts::N_SLAB_EDGES; g++) {
             BCREATE(hipStreamCreateWithFlags(&b->gst[g], hipStreamNonBlocking));
             BCREATE(hipEventCreateWithFlags(&b->ev_join[g], hipEventDisableTiming));
         }
@@ -742,7 +747,7 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     if (!auto_stride) CREATE_HIP(hipMalloc(&c->q, sizeof(double) * 2 * M * (size_t)(c->N - 1)));
     CREATE_HIP(hipMalloc(&c->T0, sizeof(double) * 2 * M));
     if (c->fused) CREATE_HIP(hipMalloc(&c->T1, sizeof(double) * 2 * M));
-    if (c->fused) CREATE_HIP(hipMalloc(&c->phacc, sizeof(double) * 4 * M));
+    if (c->fused && desc->m[0] <= 64) CREATE_HIP(hipMalloc(&c->phacc, sizeof(double) * 4 * M));  // see fused_phi
     CREATE_HIP(hipMalloc(&c->Dh, sizeof(double) * M * (size_t)(c->nhMax + 1)));
     CREATE_HIP(hipMalloc(&c->wh, sizeof(double) * M * (size_t)(c->nhMax + 1)));
     CREATE_HIP(hipMalloc(&c->noise, sizeof(double) * 2 * M));
@@ -1052,7 +1057,7 @@ int lfts_batch_destroy(lfts_batch* b) {
     for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->tune_ev)
         if (e) (void)hipEventDestroy(e);
-    for (int g = 0; g < 3; g++) {
+    for (int g = 0; g < lfts::N_SLAB_EDGES; g++) {
         if (b->gst[g]) {
             (void)hipStreamSynchronize(b->gst[g]);
             (void)hipStreamDestroy(b->gst[g]);

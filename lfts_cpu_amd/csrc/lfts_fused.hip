@@ -1,4 +1,4 @@
-// Fused contour step for cubic meshes of 32^3, 48^3 and 64^3 points: one launch per
+// Fused contour step for cubic meshes of 32^3, 48^3, 64^3 and 128^3 points: one launch per
 // step over every active replica of that mesh size.
 //
 // The contour step q_{i+1} = h ⊙ C q_i (src/step.cc:35-65) uses the 3D convolution
@@ -25,6 +25,8 @@
 // holding x[j + R2 k]. Forward: radix-R1 DFT in registers, twiddle w_n^{jk'}, exchange
 // through LDS, radix-R2 DFTs -> X[k' + R1 f]; multiply by g; the inverse retraces the
 // steps with conjugate roots. The kernel factor g includes the 1/n of each inverse.
+// Factorisations: 32 = 8 x 4, 48 = 12 x 4, 64 = 8 x 8, 128 = 16 x 8 (one 128^3 plane of
+// one field fills 149 KB of the 160 KB LDS: that size has the LDS-table shape only).
 // In LDS a line keeps element k*R2 + j at k*(R2+1) + j: the R2 threads of a line then
 // read their contiguous runs from distinct banks.
 //
@@ -200,6 +202,53 @@ __device__ __forceinline__ void dft12(double2* v) {
     }
 }
 
+// x * w16^2 and x * w16^6 (w16 = exp(-2 pi i/16) forward, its conjugate inverse)
+template <bool INV>
+__device__ __forceinline__ double2 mul_w16_2(double2 a) {
+    const double r = 0.70710678118654752440;
+    return INV ? make_double2(r * (a.x - a.y), r * (a.x + a.y)) : make_double2(r * (a.x + a.y), r * (a.y - a.x));
+}
+template <bool INV>
+__device__ __forceinline__ double2 mul_w16_6(double2 a) {
+    const double r = 0.70710678118654752440;
+    return INV ? make_double2(-r * (a.x + a.y), r * (a.x - a.y)) : make_double2(r * (a.y - a.x), -r * (a.x + a.y));
+}
+
+// natural-order radix-16 DFT as 4 x 4: x[n2 + 4 n1] -> radix-4 over n1, twiddle
+// w16^{n2 k1}, radix-4 over n2 -> X[k1 + 4 k2]
+template <bool INV>
+__device__ __forceinline__ void dft16(double2* v) {
+    const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;  // cos, sin(pi/8)
+    const double sg = INV ? 1.0 : -1.0;
+    double2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        y[n2][0] = v[n2];
+        y[n2][1] = v[n2 + 4];
+        y[n2][2] = v[n2 + 8];
+        y[n2][3] = v[n2 + 12];
+        dft4<INV>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+    }
+    const double2 w1 = make_double2(c1, sg * s1), w3 = make_double2(s1, sg * c1), w9 = make_double2(-c1, -sg * s1);
+    y[1][1] = c_mul(y[1][1], w1);
+    y[1][2] = mul_w16_2<INV>(y[1][2]);
+    y[1][3] = c_mul(y[1][3], w3);
+    y[2][1] = mul_w16_2<INV>(y[2][1]);
+    y[2][2] = c_rot<INV>(y[2][2]);  // w16^4 = -i forward
+    y[2][3] = mul_w16_6<INV>(y[2][3]);
+    y[3][1] = c_mul(y[3][1], w3);
+    y[3][2] = mul_w16_6<INV>(y[3][2]);
+    y[3][3] = c_mul(y[3][3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        dft4<INV>(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
+        v[k1] = y[0][k1];
+        v[k1 + 4] = y[1][k1];
+        v[k1 + 8] = y[2][k1];
+        v[k1 + 12] = y[3][k1];
+    }
+}
+
 // Inverse DFT of length R of v_n = y_n g_n (g real): the products are folded into the
 // first butterfly stage (pairs n, n + R/2) as fma, then the DFT continues as dft<R, true>.
 template <int R>
@@ -227,7 +276,8 @@ __device__ __forceinline__ void inv_dft_g(double2* y, const double* g) {
 
 template <int R, bool INV>
 __device__ __forceinline__ void dft(double2* v) {
-    if constexpr (R == 12) dft12<INV>(v);
+    if constexpr (R == 16) dft16<INV>(v);
+    else if constexpr (R == 12) dft12<INV>(v);
     else if constexpr (R == 8) dft8<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
     else if constexpr (R == 2) dft2<INV>(v[0], v[1]);
@@ -402,23 +452,29 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
         tabA.load(tw, ga, j);
         tabB.load(tw, gb, jb);
     }
-#pragma unroll
-    for (int k = 0; k < R1; k++) {
-        const long p = pbase + (long)(j + R2 * k) * Sa + 2 * c;
-        hz[k] = ld2(hsel + p);
-        if constexpr (EXTRA) {
-            az[k] = ld2(asel + p);
-            sz[k] = ld2(ssel + p);
-        }
-    }
-    if constexpr (PLAIN) {
+    // operands of the finish: issued now, or (derivative passes on 16-element lines,
+    // where registers are short) after the first convolution
+    constexpr bool EARLY = R1 <= 12 || (MODE == CHAIN_PLAIN && !FUSE);
+    auto load_finish = [&]() {
 #pragma unroll
         for (int k = 0; k < R1; k++) {
             const long p = pbase + (long)(j + R2 * k) * Sa + 2 * c;
-            pz[k] = ld2_nt(psel + p);
-            oz[k] = ld2_nt(accp + p);  // unused on first touch
+            hz[k] = ld2(hsel + p);
+            if constexpr (EXTRA) {
+                az[k] = ld2(asel + p);
+                sz[k] = ld2(ssel + p);
+            }
         }
-    }
+        if constexpr (PLAIN) {
+#pragma unroll
+            for (int k = 0; k < R1; k++) {
+                const long p = pbase + (long)(j + R2 * k) * Sa + 2 * c;
+                pz[k] = ld2_nt(psel + p);
+                oz[k] = ld2_nt(accp + p);  // unused on first touch
+            }
+        }
+    };
+    if constexpr (EARLY) load_finish();
     if constexpr (TWL != 0) {
         double2* stw = reinterpret_cast<double2*>(smem + FPW * BYTES);
         double* sga = reinterpret_cast<double*>(smem + FPW * BYTES + E * 16);
@@ -435,6 +491,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
 
     if constexpr (finish) {
         conv_line<R1, R2>(z, xbuf + c * XP, j, tabA);
+        if constexpr (!EARLY) load_finish();
         double* qout = io.qout;
 #pragma unroll
         for (int k = 0; k < R1; k++) {
@@ -511,7 +568,7 @@ void k_slab_io(const SlabIO io, const double2* __restrict__ tw) {
 }
 
 bool slab_supported(int n0, int n1, int n2) {
-    return n0 == n1 && n1 == n2 && (n0 == 32 || n0 == 48 || n0 == 64);
+    return n0 == n1 && n1 == n2 && (n0 == 32 || n0 == 48 || n0 == 64 || n0 == 128);
 }
 
 namespace {
@@ -547,17 +604,25 @@ void launch_one(int mode, const SlabIO& io, const double2* tw, hipStream_t st) {
 template <int E, int R1, int R2>
 void dispatch(int v, const Rep* reps, int nact, int i, int mode, const SlabIO* io, const double2* tw,
               hipStream_t st) {
-    switch (v) {
-        case 1: io ? launch_one<E, R1, R2, 1, 1>(mode, *io, tw, st) : launch_set<E, R1, R2, 1, 1>(reps, nact, i, tw, st); break;
-        case 2: io ? launch_one<E, R1, R2, 0, 2>(mode, *io, tw, st) : launch_set<E, R1, R2, 0, 2>(reps, nact, i, tw, st); break;
-        case 3: io ? launch_one<E, R1, R2, 1, 2>(mode, *io, tw, st) : launch_set<E, R1, R2, 1, 2>(reps, nact, i, tw, st); break;
-        default: io ? launch_one<E, R1, R2, 0, 1>(mode, *io, tw, st) : launch_set<E, R1, R2, 0, 1>(reps, nact, i, tw, st); break;
+    if constexpr (E == 128) {  // one shape only (see dispatch_edge)
+        (void)v;
+        io ? launch_one<E, R1, R2, 1, 1>(mode, *io, tw, st) : launch_set<E, R1, R2, 1, 1>(reps, nact, i, tw, st);
+    } else {
+        switch (v) {
+            case 1: io ? launch_one<E, R1, R2, 1, 1>(mode, *io, tw, st) : launch_set<E, R1, R2, 1, 1>(reps, nact, i, tw, st); break;
+            case 2: io ? launch_one<E, R1, R2, 0, 2>(mode, *io, tw, st) : launch_set<E, R1, R2, 0, 2>(reps, nact, i, tw, st); break;
+            case 3: io ? launch_one<E, R1, R2, 1, 2>(mode, *io, tw, st) : launch_set<E, R1, R2, 1, 2>(reps, nact, i, tw, st); break;
+            default: io ? launch_one<E, R1, R2, 0, 1>(mode, *io, tw, st) : launch_set<E, R1, R2, 0, 1>(reps, nact, i, tw, st); break;
+        }
     }
 }
 
 hipError_t dispatch_edge(int n, int v, const Rep* reps, int nact, int i, int mode, const SlabIO* io,
                          const double2* tw, hipStream_t st) {
-    if (n == 64) dispatch<64, 8, 8>(v, reps, nact, i, mode, io, tw, st);
+    // 128-point lines: 16 elements per thread leave no registers for the tables, and a
+    // plane of one field fills most of the LDS, so one shape (LDS tables, one field)
+    if (n == 128) dispatch<128, 16, 8>(1, reps, nact, i, mode, io, tw, st);
+    else if (n == 64) dispatch<64, 8, 8>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 48) dispatch<48, 12, 4>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 32) dispatch<32, 8, 4>(v, reps, nact, i, mode, io, tw, st);
     else return hipErrorInvalidValue;

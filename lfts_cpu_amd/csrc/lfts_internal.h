@@ -129,7 +129,11 @@ __host__ __device__ inline const double* rep_pair(const Rep& R, int j) {
 // i0+1..N ascending, field 1 beads N-i0..1 descending, each split by block A/B) instead
 // of storing their pairs; only pairs 1..i0 and N are stored. For odd N the middle bead
 // i0 pairs two fields of one pair and is added at the end.
-__host__ __device__ inline bool fused_phi(const Rep& R, int fuse) { return fuse && R.phacc && R.n && R.N >= 3; }
+// (Meshes up to 64^3: at 128^3 the passes fill the chip and are bandwidth-bound, so the
+// sums gain nothing there, and their registers would spill.)
+__host__ __device__ inline bool fused_phi(const Rep& R, int fuse) {
+    return fuse && R.phacc && R.n && R.n <= 64 && R.N >= 3;
+}
 // which of the four sums (field 0 A, field 0 B, field 1 A, field 1 B) receive a bead
 __host__ __device__ inline void phi_sums_used(int N, int NA, bool u[4]) {
     const int i0 = (N + 1) / 2;
@@ -177,6 +181,9 @@ __host__ __device__ inline SlabIO plain_io(const Rep& R, int i, int fuse = 0) {
 // replicas with edge n; when nact == 1 and `one` (a host copy of that descriptor) is
 // given, the descriptor travels in the kernel arguments instead.
 bool slab_supported(int n0, int n1, int n2);
+// the cubic edges the slab pass serves, largest first (one launch group each in a set)
+constexpr int SLAB_EDGES[] = {128, 64, 48, 32};
+constexpr int N_SLAB_EDGES = 4;
 // `variant` selects one of SLAB_VARIANTS kernel shapes (bit 0: twiddle/kernel tables in
 // LDS instead of registers; bit 1: both fields of a plane in one workgroup). All variants
 // give bit-identical results; the fastest one per mesh size and set size is picked by

@@ -33,7 +33,7 @@ struct lfts_ctx {
     long M = 0, Mk = 0;
     int N = 0, NA = 0, NB = 0, nhMax = 10;
     lfts_abi::AxisPlan ax[3];
-    bool fused = false;       // cubic 32/48/64 mesh: one slab launch per contour step
+    bool fused = false;       // cubic 32/48/64/128 mesh: one slab launch per contour step
     double* w = nullptr;      // 4M
     double* h = nullptr;      // 2M (pair 1)
     double* q = nullptr;      // (N-1) pairs
@@ -102,9 +102,9 @@ struct lfts_batch {
     std::vector<int> line_reps;
     // one stream per slab group, so the groups of a contour step overlap on the GPU;
     // forked from and joined back to `st` around each propagator chain
-    hipStream_t gst[3] = {nullptr, nullptr, nullptr};
+    hipStream_t gst[lfts::N_SLAB_EDGES] = {};
     hipEvent_t ev_fork = nullptr;
-    hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_join[lfts::N_SLAB_EDGES] = {};
     // slab kernel variant per (mesh edge, members in the launch), picked by timing
     std::map<std::pair<int, int>, int> tuned;
     std::string tune_report;
