@@ -85,3 +85,34 @@ def test_large_case_langevin_step_both_paths(gpu, monkeypatch):
     assert abs(qa - qb) <= 1e-8 * max(1.0, abs(qb))
     assert rel(wa[:M], wb[:M]) < 1e-8 and rel(wa[M:2 * M], wb[M:2 * M]) < 1e-8
     assert rel(wa[3 * M:], wb[3 * M:]) < 1e-8
+
+
+def test_set_with_a_128_member_equals_members_alone(gpu):
+    """A replica set spanning all four slab launch groups (128^3, 64^3, 48^3, 32^3) plus a
+    line-path member gives every member's Q and fields bit for bit as it gets alone."""
+    specs = [((128,) * 3, 6, 2), ((64,) * 3, 8, 3), ((48,) * 3, 7, 4), ((32,) * 3, 9, 5), ((20, 18, 16), 6, 3)]
+
+    def make():
+        cs = []
+        for k, (m, n, na) in enumerate(specs):
+            c = L.Context(n, na, m, (4.1, 4.0, 3.9))
+            M = int(np.prod(m))
+            rng = np.random.default_rng(40 + k)
+            c.set_fields(np.concatenate([rng.normal(0, 1.0, M), rng.normal(0, 0.5, M)]))
+            cs.append(c)
+        return cs
+
+    print("set of 5 with a 128^3 member", flush=True)
+    ctxs = make()
+    b = L.Batch(ctxs)
+    lnq = b.calc_concs()
+    k, err = b.anderson_mix(8, 1e-12)
+    got = [c.get_fields(4) for c in ctxs]
+    b.close()
+    print("members alone", flush=True)
+    for i, c in enumerate(make()):
+        assert c.calc_concs() == lnq[i], i
+        ki, erri = c.anderson_mix(8, 1e-12)
+        assert ki == k[i] and erri == err[i], i
+        assert np.array_equal(c.get_fields(4), got[i]), i
+        c.close()
