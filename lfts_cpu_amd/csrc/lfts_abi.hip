@@ -454,18 +454,27 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
     std::vector<int> active;
     for (int id = 0; id < n; id++)
         if (1 < maxIter && err[id] > errTol) active.push_back(id);
+    // One coefficient upload per iteration: the one after the host LU also carries the
+    // history depth of the next iteration's Gram row (the first iteration's is 0). The
+    // pinned coefficients are rewritten only after read_scal has synchronised, so the
+    // next evaluation is enqueued while the update still runs, and the descriptors are
+    // uploaded again only when the active set changes.
+    if (active.empty()) goto done;
+    for (int id : active) b->h_coef[id].nh_dots = 0;
+    if (int rc = upload_coef(b)) return rc;
+    {
+    std::vector<int> cur;
     while (!active.empty()) {
-        if (int rc = set_active(b, active)) return rc;
+        if (active != cur) {
+            if (int rc = set_active(b, active)) return rc;
+            cur = active;
+        }
         if (int rc = calc_concs_async(b)) return rc;
         int nh_max = 0;
         for (int id : active) {
-            lfts_ctx* c = b->ctx[id];
-            Coef& co = b->h_coef[id];
-            co.nh = (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax;
-            co.nh_dots = co.nh;
-            nh_max = std::max(nh_max, co.nh);
+            const lfts_ctx* c = b->ctx[id];
+            nh_max = std::max(nh_max, (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax);
         }
-        if (int rc = upload_coef(b)) return rc;
         const int nact = (int)b->act.size();
         BHIPCHK(b, lfts::launch_am_dots(b->d_ract, nact, b->nb_max, b->d_coef, b->d_part, b->st));
         BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, nh_max + 1, 0, b->d_scal, 0, b->st));
@@ -475,7 +484,10 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             const int H = c->nhMax + 1;
             auto DD = [&](int i, int j) -> double& { return c->DD[(size_t)i * H + j]; };
             Coef& co = b->h_coef[id];
-            const int nh = co.nh;
+            const int nh = (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax;
+            co.nh = nh;
+            // the next iteration's Gram row (k + 1)
+            co.nh_dots = (k[id] + 1 < c->nhMax + 1) ? k[id] : c->nhMax;
             const double S1 = scal(b, id, 0);
             err[id] = std::pow(S1 / c->M, 0.5);
             for (int m = 0; m <= nh; m++) DD(0, m) = scal(b, id, m);
@@ -509,10 +521,11 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             k[id]++;
             if (k[id] < maxIter && err[id] > errTol) still.push_back(id);
         }
-        // the update must finish before the pinned lists are rewritten
-        if (int rc = sync_batch(b)) return rc;
         active.swap(still);
     }
+    }
+    if (int rc = sync_batch(b)) return rc;
+done:
     bool nonfinite = false;
     for (int id = 0; id < n; id++) {
         if (k_ret) k_ret[id] = k[id];

@@ -31,6 +31,26 @@ __device__ double block_sum(double v, double* red) {
     return s;
 }
 
+// block_sum of nc components at once (component c's result in out[c], written by thread
+// c): each component takes exactly block_sum's operations, with one barrier for all.
+template <int NC>
+__device__ void block_sums(double (&v)[NC], int nc, double* red, double* out) {
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+        if (c < nc) v[c] = wave_sum(v[c]);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+            if (c < nc) red[wv * NC + c] = v[c];
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        double s = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i * NC + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
 __device__ __forceinline__ const double* pair_ptr(const Rep& R, int i) {
     return (i == 1) ? R.h : R.q + 2L * (i - 2) * R.M;
 }
@@ -67,19 +87,27 @@ __global__ __launch_bounds__(256) void k_q_sum(const Rep* __restrict__ reps, dou
     if (threadIdx.x == 0) part_row(part, blockIdx.y)[0] = s;
 }
 
-// scal[rid][off + c] = sum over the replica's partials (divided by M when asked)
+// scal[rid][off + c] = sum over the replica's partials (divided by M when asked), all
+// ncomp (<= PSTRIDE) components together
 __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, const double* __restrict__ part,
                                                 int ncomp, int div_by_M, double* __restrict__ scal, int off) {
-    __shared__ double red[8];
+    __shared__ double red[4 * PSTRIDE];
+    __shared__ double res[PSTRIDE];
     const int slot = blockIdx.x;
     const Rep R = reps[slot];
     if (R.skip && *R.skip) return;
     const double* p = part + (long)slot * MAX_NB * PSTRIDE;
-    for (int c = 0; c < ncomp; c++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < R.nb; b += blockDim.x) s += p[(long)b * PSTRIDE + c];
-        s = block_sum(s, red);
-        if (threadIdx.x == 0) scal[(long)R.id * SSTRIDE + off + c] = div_by_M ? s / R.M : s;
+    double s[PSTRIDE];
+#pragma unroll
+    for (int c = 0; c < PSTRIDE; c++) s[c] = 0.0;
+    for (int b = threadIdx.x; b < R.nb; b += blockDim.x)
+#pragma unroll
+        for (int c = 0; c < PSTRIDE; c++)
+            if (c < ncomp) s[c] += p[(long)b * PSTRIDE + c];
+    block_sums<PSTRIDE>(s, ncomp, red, res);
+    if ((int)threadIdx.x < ncomp) {
+        const double v = res[threadIdx.x];
+        scal[(long)R.id * SSTRIDE + off + threadIdx.x] = div_by_M ? v / R.M : v;
     }
 }
 
@@ -134,7 +162,7 @@ __global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const
 // src/anderson.cc:35-55: DD[0][m] = <Dh0, Dh_m>, Dh0 = phi+ - 1 (m = 0 uses Dh0 itself)
 __global__ __launch_bounds__(256) void k_am_dots(const Rep* __restrict__ reps, const Coef* __restrict__ coef,
                                                  double* __restrict__ part) {
-    __shared__ double red[8];
+    __shared__ double redn[4 * (MAX_HIST + 1)];
     REPLICA(R);
     const int nh = coef[R.id].nh_dots;
     const long M = R.M;
@@ -148,13 +176,7 @@ __global__ __launch_bounds__(256) void k_am_dots(const Rep* __restrict__ reps, c
         for (int m = 1; m <= MAX_HIST; m++)
             if (m <= nh) acc[m] += d0 * R.Dh[(long)m * M + r];
     }
-    double* out = part_row(part, blockIdx.y);
-#pragma unroll
-    for (int m = 0; m <= MAX_HIST; m++) {
-        if (m > nh) break;
-        const double s = block_sum(acc[m], red);
-        if (threadIdx.x == 0) out[m] = s;
-    }
+    block_sums<MAX_HIST + 1>(acc, nh + 1, redn, part_row(part, blockIdx.y));
 }
 
 // src/anderson.cc:57-89: w+ += lambda Dh0 + sum_n C_n((wh_n + lambda Dh_n) - (wh0 + lambda Dh0)),
