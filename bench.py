@@ -32,6 +32,7 @@ if ROOT not in sys.path:
 METRIC = "Langevin steps/s at 64³,N=90 FP64 (plain vs batched); % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
 NK_STEPS = 10  # Langevin steps timed with the Newton-Krylov solver in the derivative leg
+HOST_STEPS = 5  # Langevin steps with host noise (PCIe-inclusive rate)
 PROF_STEPS = 3  # steps after the timed region with live event timing of the contour chains
 
 
@@ -380,6 +381,20 @@ def main():
         adjoint = adjoint_leg(ctx, eta[e0].data_ptr(), [eta[e0 + 1 + k] for k in range(NK_STEPS)], XbN, dt, M, torch)
         log(f"adjoint leg: {adjoint}")
 
+    # PCIe-inclusive rate (reported, never `value`): the same step with the noise handed
+    # over in host memory, as the lfts-gpu driver does it (its noise generation runs on a
+    # host thread meanwhile; here it is drawn beforehand)
+    host_eta = [rng.normals(M) for _ in range(HOST_STEPS)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for e in host_eta:
+        ctx.langevin_step(e, XbN, dt)
+        ctx.anderson_mix(200, 1e-4)
+        ctx.calc_concs()
+    torch.cuda.synchronize()
+    host_rate = HOST_STEPS / (time.perf_counter() - t1)
+    log(f"host-noise steps: {host_rate:.2f} steps/s")
+
     batched = None
     if args.batched_steps > 0:
         ctx.close()
@@ -422,6 +437,7 @@ def main():
         "step_roofline_frac": step_gbps / HBM_PEAK_GBPS,
         "contour_steps_M_per_s": cs_per_s,
         "propagator_eval_ms": ms_pe / max(n_pe, 1),
+        "pcie_inclusive_steps_per_s": host_rate,
         "cpu_baseline": cpu,
         "batched": batched,
         "newton_krylov": adjoint,
