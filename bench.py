@@ -343,13 +343,16 @@ def main():
     torch.cuda.synchronize()
 
     nAM = []
+    marks = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.warmup, nsteps):
-        lnq, k = ctx.full_step_dev(eta[s].data_ptr(), XbN, dt)
+        lnq, k = ctx.full_step_dev(eta[s].data_ptr(), XbN, dt)  # returns after the step's final read-back
+        marks.append(time.perf_counter())
         nAM.append(k - 1)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    step_ms = np.diff(np.array([t0] + marks)) * 1e3
     log(f"timed {args.steps} steps: {elapsed:.3f} s, n_AM={nAM}, lnQ={lnq:.10g}")
     # a few more steps with live HIP-event timing of the contour chains (the roofline
     # figure), outside the timed region
@@ -380,6 +383,19 @@ def main():
         e0 = nsteps + PROF_STEPS
         adjoint = adjoint_leg(ctx, eta[e0].data_ptr(), [eta[e0 + 1 + k] for k in range(NK_STEPS)], XbN, dt, M, torch)
         log(f"adjoint leg: {adjoint}")
+
+    # attainable HBM bandwidth: a device-to-device copy of 1 GiB (read + write counted)
+    xs = torch.empty(2 ** 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    ys = torch.empty_like(xs)
+    ys.copy_(xs)
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    for _ in range(5):
+        ys.copy_(xs)
+    torch.cuda.synchronize()
+    copy_gbps = 5 * 2 * xs.numel() * 8 / (time.perf_counter() - c0) / 1e9
+    del xs, ys
+    log(f"copy bandwidth: {copy_gbps:.0f} GB/s")
 
     # PCIe-inclusive rate (reported, never `value`): the same step with the noise handed
     # over in host memory, as the lfts-gpu driver does it (its noise generation runs on a
@@ -431,8 +447,12 @@ def main():
             "kernel": "contour step (48*M algorithmic bytes; plain and fused-density-sum passes, 44:45 per "
                       f"evaluation), HIP events on the context stream over {PROF_STEPS} further steps",
             "contour_step_us": cs_ms * 1e3,
+            "attainable_copy_GBps": copy_gbps,
             "slab_tuning": tuning_plain,
         },
+        "step_ms_median": float(np.median(step_ms)),
+        "steps_per_s_median": float(1e3 / np.median(step_ms)),
+        "n_AM_histogram": {str(v): int(c) for v, c in zip(*np.unique(np.array(nAM), return_counts=True))},
         "step_algorithmic_GBps": step_gbps,
         "step_roofline_frac": step_gbps / HBM_PEAK_GBPS,
         "contour_steps_M_per_s": cs_per_s,
