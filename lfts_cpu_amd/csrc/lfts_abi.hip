@@ -538,9 +538,9 @@ done:
 // ---- Anderson mixing on the device, one context -----------------------------------------
 
 // The device-resident loop is chosen with LFTS_ANDERSON=device. It is not the default:
-// at 64^3, N=90 the host loop (one synchronisation and a host LU per iteration) measured
-// 48.2 ms per Langevin step against 49.6 ms for the device loop (its two extra kernels
-// per iteration, the flag test in every kernel and the speculative tail cost more than the
+// at 64^3, N=90 the host loop (one read-back and a host LU per iteration) measured 26.95
+// Langevin steps/s against 26.13 for the device loop (its two extra kernels per
+// iteration, the flag test in every kernel and the speculative tail cost more than the
 // round trip saves); graph replay of an iteration made no measurable difference.
 bool device_anderson_enabled(const lfts_ctx* c) {
     const char* e = std::getenv("LFTS_ANDERSON");
