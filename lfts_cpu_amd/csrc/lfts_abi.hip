@@ -459,10 +459,10 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
     // pinned coefficients are rewritten only after read_scal has synchronised, so the
     // next evaluation is enqueued while the update still runs, and the descriptors are
     // uploaded again only when the active set changes.
-    if (active.empty()) goto done;
-    for (int id : active) b->h_coef[id].nh_dots = 0;
-    if (int rc = upload_coef(b)) return rc;
-    {
+    if (!active.empty()) {
+        for (int id : active) b->h_coef[id].nh_dots = 0;
+        if (int rc = upload_coef(b)) return rc;
+    }
     std::vector<int> cur;
     while (!active.empty()) {
         if (active != cur) {
@@ -523,9 +523,7 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
         }
         active.swap(still);
     }
-    }
     if (int rc = sync_batch(b)) return rc;
-done:
     bool nonfinite = false;
     for (int id = 0; id < n; id++) {
         if (k_ret) k_ret[id] = k[id];
