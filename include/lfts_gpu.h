@@ -114,8 +114,11 @@ int lfts_hamiltonian(lfts_ctx* ctx, double XbN, double* H);
 int lfts_set_pair_storage(lfts_ctx* ctx, int stride);
 int lfts_pair_storage(lfts_ctx* ctx, int* stride, int* nkept);
 
-/* Testing access: stored propagator pair i (1..N) as the reference lays it out,
- * [q_i | q+_{N+1-i}] (src/diblock.cc:19-24), 2*M host doubles. */
+/* Testing access: stored propagator pair i (1..N) of the last evaluation as the
+ * reference lays it out, [q_i | q+_{N+1-i}] (src/diblock.cc:19-24), 2*M host doubles.
+ * lfts_calc_concs keeps every pair. On one context with a cubic mesh up to 64^3, an
+ * Anderson iteration or lfts_hamiltonian forms the density sums during the chain and
+ * keeps only pairs 1..ceil(N/2) and N; another i then returns LFTS_EINVAL. */
 int lfts_get_pair(lfts_ctx* ctx, int i, double* out);
 /* step::fwd(q_in, q_out, h, i) (src/step.cc:35-65) on host arrays (2*M each). */
 int lfts_step_fwd(lfts_ctx* ctx, const double* q_in, double* q_out, const double* h, int i);
