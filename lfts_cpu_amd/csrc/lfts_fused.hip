@@ -1,4 +1,4 @@
-// Fused contour step for cubic meshes of 32^3, 48^3, 64^3 and 128^3 points: one launch per
+// Fused contour step for cubic meshes of 32^3, 40^3, 48^3, 64^3 and 128^3 points: one launch per
 // step over every active replica of that mesh size.
 //
 // The contour step q_{i+1} = h ⊙ C q_i (src/step.cc:35-65) uses the 3D convolution
@@ -25,8 +25,9 @@
 // holding x[j + R2 k]. Forward: radix-R1 DFT in registers, twiddle w_n^{jk'}, exchange
 // through LDS, radix-R2 DFTs -> X[k' + R1 f]; multiply by g; the inverse retraces the
 // steps with conjugate roots. The kernel factor g includes the 1/n of each inverse.
-// Factorisations: 32 = 8 x 4, 48 = 12 x 4, 64 = 8 x 8, 128 = 16 x 8 (one 128^3 plane of
-// one field fills 149 KB of the 160 KB LDS: that size has the LDS-table shape only).
+// Factorisations: 32 = 8 x 4, 40 = 5 x 8 (five stage-2 DFTs over eight threads: three
+// idle in that stage), 48 = 12 x 4, 64 = 8 x 8, 128 = 16 x 8 (one 128^3 plane of one field
+// fills 149 KB of the 160 KB LDS: that size has the LDS-table shape only).
 // In LDS a line keeps element k*R2 + j at k*(R2+1) + j: the R2 threads of a line then
 // read their contiguous runs from distinct banks.
 //
@@ -155,6 +156,30 @@ __device__ __forceinline__ void dft8(double2* v) {
     dft8_tail<INV>(a, b, v);
 }
 
+// natural-order radix-5 DFT: with a = x1 + x4, b = x2 + x3, c = x1 - x4, d = x2 - x3,
+// X1,4 = x0 + c1 a + c2 b -/+ i (s1 c + s2 d) and X2,3 = x0 + c2 a + c1 b -/+ i (s2 c - s1 d)
+// (forward; the inverse swaps the signs of the i terms), c_k, s_k = cos, sin(2 pi k / 5)
+template <bool INV>
+__device__ __forceinline__ void dft5(double2* v) {
+    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+    const double2 a = c_add(v[1], v[4]), b = c_add(v[2], v[3]);
+    const double2 c = c_sub(v[1], v[4]), d = c_sub(v[2], v[3]);
+    const double2 x0 = v[0];
+    const double2 t1 = make_double2(fma(c2, b.x, fma(c1, a.x, x0.x)), fma(c2, b.y, fma(c1, a.y, x0.y)));
+    const double2 t2 = make_double2(fma(c1, b.x, fma(c2, a.x, x0.x)), fma(c1, b.y, fma(c2, a.y, x0.y)));
+    const double2 u1 = make_double2(fma(s2, d.x, s1 * c.x), fma(s2, d.y, s1 * c.y));
+    const double2 u2 = make_double2(fma(-s1, d.x, s2 * c.x), fma(-s1, d.y, s2 * c.y));
+    v[0] = c_add(x0, c_add(a, b));
+    // t - i u = (t.x + u.y, t.y - u.x); t + i u = (t.x - u.y, t.y + u.x)
+    const double2 t1m = make_double2(t1.x + u1.y, t1.y - u1.x), t1p = make_double2(t1.x - u1.y, t1.y + u1.x);
+    const double2 t2m = make_double2(t2.x + u2.y, t2.y - u2.x), t2p = make_double2(t2.x - u2.y, t2.y + u2.x);
+    v[1] = INV ? t1p : t1m;
+    v[4] = INV ? t1m : t1p;
+    v[2] = INV ? t2p : t2m;
+    v[3] = INV ? t2m : t2p;
+}
+
 // natural-order radix-3 DFT; the 1/2 and sqrt(3)/2 products are folded into their adds
 template <bool INV>
 __device__ __forceinline__ void dft3(double2& a, double2& b, double2& c) {
@@ -279,9 +304,15 @@ __device__ __forceinline__ void dft(double2* v) {
     if constexpr (R == 16) dft16<INV>(v);
     else if constexpr (R == 12) dft12<INV>(v);
     else if constexpr (R == 8) dft8<INV>(v);
+    else if constexpr (R == 5) dft5<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
     else if constexpr (R == 2) dft2<INV>(v[0], v[1]);
 }
+
+// Stage-2 DFTs (of length R2) per thread: R1 / R2, or, when R1 < R2, one for threads
+// j < R1 and none for the others.
+template <int R1, int R2>
+constexpr int stage2_per_thread() { return R1 >= R2 ? R1 / R2 : 1; }
 
 // Twiddles w_n^{jk} (k < R1) and kernel values g(jU + u + R1 f) of the line lane j a
 // thread serves: held in registers (loaded once per pass) or read from the workgroup's
@@ -289,16 +320,17 @@ __device__ __forceinline__ void dft(double2* v) {
 // run two waves per SIMD).
 template <int R1, int R2>
 struct RegTab {
+    static constexpr int U = stage2_per_thread<R1, R2>();
     double2 tw[R1];
-    double g[R1];
+    double g[U * R2];
     __device__ __forceinline__ void load(const double2* __restrict__ twt, const double* __restrict__ gt, int j) {
-        constexpr int U = R1 / R2;
+        const int jg = (R1 >= R2 || j < R1) ? j : 0;  // lanes without a stage-2 DFT read lane 0's
 #pragma unroll
         for (int k = 0; k < R1; k++) tw[k] = ld2(twt + j * k);
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int f = 0; f < R2; f++) g[u * R2 + f] = ldg(gt + j * U + u + R1 * f);
+            for (int f = 0; f < R2; f++) g[u * R2 + f] = ldg(gt + jg * U + u + R1 * f);
     }
     __device__ __forceinline__ double2 w(int k) const { return tw[k]; }
     __device__ __forceinline__ double gv(int u, int f) const { return g[u * R2 + f]; }
@@ -315,7 +347,9 @@ struct LdsTab {
         j = jj;
     }
     __device__ __forceinline__ double2 w(int k) const { return tw[j * k]; }
-    __device__ __forceinline__ double gv(int u, int f) const { return g[j * (R1 / R2) + u + R1 * f]; }
+    __device__ __forceinline__ double gv(int u, int f) const {
+        return g[j * stage2_per_thread<R1, R2>() + u + R1 * f];
+    }
 };
 
 // 1D circular convolution of one complex line of length N = R1*R2 held by R2 threads
@@ -323,8 +357,9 @@ struct LdsTab {
 // (R1*R2 complex); tab supplies the thread's twiddles exp(-2 pi i jk/N) and kernel values.
 template <int R1, int R2, class Tab>
 __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const Tab& tab) {
-    constexpr int U = R1 / R2;
+    constexpr int U = stage2_per_thread<R1, R2>();
     constexpr int Q = R2 + 1;  // padded run length in LDS
+    const bool stage2 = R1 >= R2 || j < R1;
     dft<R1, false>(z);
 #pragma unroll
     for (int k = 1; k < R1; k++) z[k] = c_mul(z[k], tab.w(k));
@@ -332,24 +367,28 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 #pragma unroll
     for (int k = 0; k < R1; k++) lx[k * Q + j] = z[k];
     lds_barrier();
-    double2 y[R1];
+    double2 y[U * R2];
+    if (stage2) {
 #pragma unroll
-    for (int u = 0; u < U; u++)
+        for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int jj = 0; jj < R2; jj++) y[u * R2 + jj] = lx[(j * U + u) * Q + jj];
+            for (int jj = 0; jj < R2; jj++) y[u * R2 + jj] = lx[(j * U + u) * Q + jj];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        dft<R2, false>(y + u * R2);
-        double gl[R2];
+        for (int u = 0; u < U; u++) {
+            dft<R2, false>(y + u * R2);
+            double gl[R2];
 #pragma unroll
-        for (int f = 0; f < R2; f++) gl[f] = tab.gv(u, f);
-        inv_dft_g<R2>(y + u * R2, gl);
+            for (int f = 0; f < R2; f++) gl[f] = tab.gv(u, f);
+            inv_dft_g<R2>(y + u * R2, gl);
+        }
     }
     lds_barrier();
+    if (stage2) {
 #pragma unroll
-    for (int u = 0; u < U; u++)
+        for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int jj = 0; jj < R2; jj++) lx[(j * U + u) * Q + jj] = y[u * R2 + jj];
+            for (int jj = 0; jj < R2; jj++) lx[(j * U + u) * Q + jj] = y[u * R2 + jj];
+    }
     lds_barrier();
 #pragma unroll
     for (int k = 0; k < R1; k++) z[k] = lx[k * Q + j];
@@ -381,11 +420,12 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 // the finish would make it wait for every store before the next convolution).
 template <int E, int R1, int R2, int MODE, int TWL, int FPW, int FUSE, int FIN>
 __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const double2* __restrict__ tw) {
-    static_assert(R1 * R2 == E && R1 % R2 == 0, "unsupported factorisation");
+    static_assert(R1 * R2 == E && (R1 % R2 == 0 || R1 < R2), "unsupported factorisation");
     constexpr int H = E / 2;           // complex lines per plane
     constexpr int TPF = H * R2;        // threads per field
-    constexpr int XP = R1 * (R2 + 1) + 1;  // exchange row pitch (complex): padded line + 1
-    constexpr int RP = E + 8;          // real plane row pitch (doubles): rows r, r+1 on other banks
+    constexpr int XP = (R1 * (R2 + 1)) | 1;  // exchange row pitch (complex): padded line, odd
+    constexpr int RP = E + ((E + 8) % 16 == 8 ? 8 : 16);  // real plane row pitch (doubles), = 8 mod 16:
+                                                          // rows r, r+1 on other banks
     constexpr int XBYTES = H * XP * 16;
     constexpr int RBYTES = E * RP * 8;
     constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
@@ -568,7 +608,7 @@ void k_slab_io(const SlabIO io, const double2* __restrict__ tw) {
 }
 
 bool slab_supported(int n0, int n1, int n2) {
-    return n0 == n1 && n1 == n2 && (n0 == 32 || n0 == 48 || n0 == 64 || n0 == 128);
+    return n0 == n1 && n1 == n2 && (n0 == 32 || n0 == 40 || n0 == 48 || n0 == 64 || n0 == 128);
 }
 
 namespace {
@@ -623,6 +663,7 @@ hipError_t dispatch_edge(int n, int v, const Rep* reps, int nact, int i, int mod
     // plane of one field fills most of the LDS, so one shape (LDS tables, one field)
     if (n == 128) dispatch<128, 16, 8>(1, reps, nact, i, mode, io, tw, st);
     else if (n == 64) dispatch<64, 8, 8>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 40) dispatch<40, 5, 8>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 48) dispatch<48, 12, 4>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 32) dispatch<32, 8, 4>(v, reps, nact, i, mode, io, tw, st);
     else return hipErrorInvalidValue;

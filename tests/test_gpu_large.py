@@ -88,9 +88,10 @@ def test_large_case_langevin_step_both_paths(gpu, monkeypatch):
 
 
 def test_set_with_a_128_member_equals_members_alone(gpu):
-    """A replica set spanning all four slab launch groups (128^3, 64^3, 48^3, 32^3) plus a
-    line-path member gives every member's Q and fields bit for bit as it gets alone."""
-    specs = [((128,) * 3, 6, 2), ((64,) * 3, 8, 3), ((48,) * 3, 7, 4), ((32,) * 3, 9, 5), ((20, 18, 16), 6, 3)]
+    """A replica set spanning all five slab launch groups (128^3, 64^3, 48^3, 40^3, 32^3)
+    plus a line-path member gives every member's Q and fields bit for bit as it gets alone."""
+    specs = [((128,) * 3, 6, 2), ((64,) * 3, 8, 3), ((48,) * 3, 7, 4), ((40,) * 3, 8, 5), ((32,) * 3, 9, 5),
+             ((20, 18, 16), 6, 3)]
 
     def make():
         cs = []
