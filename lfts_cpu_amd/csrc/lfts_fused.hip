@@ -1,5 +1,5 @@
-// Fused contour step for cubic meshes of 32^3, 40^3, 48^3, 64^3 and 128^3 points: one launch per
-// step over every active replica of that mesh size.
+// Fused contour step for cubic meshes of 16^3, 32^3, 40^3, 48^3, 64^3, 96^3 and 128^3
+// points: one launch per step over every active replica of that mesh size.
 //
 // The contour step q_{i+1} = h ⊙ C q_i (src/step.cc:35-65) uses the 3D convolution
 // C = Cx Cy Cz, separable into 1D circular convolutions along x, y and z. A slab of the
@@ -25,9 +25,10 @@
 // holding x[j + R2 k]. Forward: radix-R1 DFT in registers, twiddle w_n^{jk'}, exchange
 // through LDS, radix-R2 DFTs -> X[k' + R1 f]; multiply by g; the inverse retraces the
 // steps with conjugate roots. The kernel factor g includes the 1/n of each inverse.
-// Factorisations: 32 = 8 x 4, 40 = 5 x 8 (five stage-2 DFTs over eight threads: three
-// idle in that stage), 48 = 12 x 4, 64 = 8 x 8, 128 = 16 x 8 (one 128^3 plane of one field
-// fills 149 KB of the 160 KB LDS: that size has the LDS-table shape only).
+// Factorisations: 16 = 4 x 4, 32 = 8 x 4, 40 = 5 x 8 (five stage-2 DFTs over eight threads:
+// three idle in that stage), 48 = 12 x 4, 64 = 8 x 8, 96 = 12 x 8 (twelve stage-2 DFTs over
+// eight threads; one plane fills 87 KB of LDS), 128 = 16 x 8 (one plane fills 149 KB of the
+// 160 KB LDS). 96 and 128 have the LDS-table, one-field shape only.
 // In LDS a line keeps element k*R2 + j at k*(R2+1) + j: the R2 threads of a line then
 // read their contiguous runs from distinct banks.
 //
@@ -309,10 +310,15 @@ __device__ __forceinline__ void dft(double2* v) {
     else if constexpr (R == 2) dft2<INV>(v[0], v[1]);
 }
 
-// Stage-2 DFTs (of length R2) per thread: R1 / R2, or, when R1 < R2, one for threads
-// j < R1 and none for the others.
+// Stage-2 DFTs (of length R2, R1 of them) per thread: up to ceil(R1 / R2). When R2
+// divides R1, thread j takes DFTs j U .. j U + U - 1; otherwise DFT u R2 + j for each u
+// with u R2 + j < R1 (some threads take one fewer, or none).
 template <int R1, int R2>
-constexpr int stage2_per_thread() { return R1 >= R2 ? R1 / R2 : 1; }
+constexpr int stage2_per_thread() { return (R1 + R2 - 1) / R2; }
+template <int R1, int R2>
+__device__ __forceinline__ int stage2_dft(int j, int u) {
+    return (R1 % R2 == 0) ? j * (R1 / R2) + u : u * R2 + j;
+}
 
 // Twiddles w_n^{jk} (k < R1) and kernel values g(jU + u + R1 f) of the line lane j a
 // thread serves: held in registers (loaded once per pass) or read from the workgroup's
@@ -324,13 +330,15 @@ struct RegTab {
     double2 tw[R1];
     double g[U * R2];
     __device__ __forceinline__ void load(const double2* __restrict__ twt, const double* __restrict__ gt, int j) {
-        const int jg = (R1 >= R2 || j < R1) ? j : 0;  // lanes without a stage-2 DFT read lane 0's
 #pragma unroll
         for (int k = 0; k < R1; k++) tw[k] = ld2(twt + j * k);
 #pragma unroll
-        for (int u = 0; u < U; u++)
+        for (int u = 0; u < U; u++) {
+            const int d0 = stage2_dft<R1, R2>(j, u);
+            const int d = d0 < R1 ? d0 : 0;  // a slot without a DFT reads DFT 0's values
 #pragma unroll
-            for (int f = 0; f < R2; f++) g[u * R2 + f] = ldg(gt + jg * U + u + R1 * f);
+            for (int f = 0; f < R2; f++) g[u * R2 + f] = ldg(gt + d + R1 * f);
+        }
     }
     __device__ __forceinline__ double2 w(int k) const { return tw[k]; }
     __device__ __forceinline__ double gv(int u, int f) const { return g[u * R2 + f]; }
@@ -347,9 +355,7 @@ struct LdsTab {
         j = jj;
     }
     __device__ __forceinline__ double2 w(int k) const { return tw[j * k]; }
-    __device__ __forceinline__ double gv(int u, int f) const {
-        return g[j * stage2_per_thread<R1, R2>() + u + R1 * f];
-    }
+    __device__ __forceinline__ double gv(int u, int f) const { return g[stage2_dft<R1, R2>(j, u) + R1 * f]; }
 };
 
 // 1D circular convolution of one complex line of length N = R1*R2 held by R2 threads
@@ -359,7 +365,6 @@ template <int R1, int R2, class Tab>
 __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const Tab& tab) {
     constexpr int U = stage2_per_thread<R1, R2>();
     constexpr int Q = R2 + 1;  // padded run length in LDS
-    const bool stage2 = R1 >= R2 || j < R1;
     dft<R1, false>(z);
 #pragma unroll
     for (int k = 1; k < R1; k++) z[k] = c_mul(z[k], tab.w(k));
@@ -368,13 +373,12 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
     for (int k = 0; k < R1; k++) lx[k * Q + j] = z[k];
     lds_barrier();
     double2 y[U * R2];
-    if (stage2) {
 #pragma unroll
-        for (int u = 0; u < U; u++)
+    for (int u = 0; u < U; u++) {
+        const int d = stage2_dft<R1, R2>(j, u);
+        if (R1 % R2 == 0 || d < R1) {
 #pragma unroll
-            for (int jj = 0; jj < R2; jj++) y[u * R2 + jj] = lx[(j * U + u) * Q + jj];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
+            for (int jj = 0; jj < R2; jj++) y[u * R2 + jj] = lx[d * Q + jj];
             dft<R2, false>(y + u * R2);
             double gl[R2];
 #pragma unroll
@@ -383,11 +387,12 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
         }
     }
     lds_barrier();
-    if (stage2) {
 #pragma unroll
-        for (int u = 0; u < U; u++)
+    for (int u = 0; u < U; u++) {
+        const int d = stage2_dft<R1, R2>(j, u);
+        if (R1 % R2 == 0 || d < R1)
 #pragma unroll
-            for (int jj = 0; jj < R2; jj++) lx[(j * U + u) * Q + jj] = y[u * R2 + jj];
+            for (int jj = 0; jj < R2; jj++) lx[d * Q + jj] = y[u * R2 + jj];
     }
     lds_barrier();
 #pragma unroll
@@ -420,7 +425,7 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 // the finish would make it wait for every store before the next convolution).
 template <int E, int R1, int R2, int MODE, int TWL, int FPW, int FUSE, int FIN>
 __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const double2* __restrict__ tw) {
-    static_assert(R1 * R2 == E && (R1 % R2 == 0 || R1 < R2), "unsupported factorisation");
+    static_assert(R1 * R2 == E, "unsupported factorisation");
     constexpr int H = E / 2;           // complex lines per plane
     constexpr int TPF = H * R2;        // threads per field
     constexpr int XP = (R1 * (R2 + 1)) | 1;  // exchange row pitch (complex): padded line, odd
@@ -608,7 +613,10 @@ void k_slab_io(const SlabIO io, const double2* __restrict__ tw) {
 }
 
 bool slab_supported(int n0, int n1, int n2) {
-    return n0 == n1 && n1 == n2 && (n0 == 32 || n0 == 40 || n0 == 48 || n0 == 64 || n0 == 128);
+    if (n0 != n1 || n1 != n2) return false;
+    for (int e : SLAB_EDGES)
+        if (n0 == e) return true;
+    return false;
 }
 
 namespace {
@@ -624,9 +632,16 @@ template <int E, int R1, int R2, int TWL, int FPW>
 void launch_one(int mode, const SlabIO& io, const double2* tw, hipStream_t st) {
     const dim3 grid(E, 2 / FPW), block((E / 2) * R2 * FPW);
     const bool fin = io.finish != 0;
-    if (mode == CHAIN_PLAIN && io.acc[0] && fin)
-        hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 1, 1>), grid, block, 0, st, io, tw);
-    else if (mode == CHAIN_PLAIN && fin)
+    bool fused = false;
+    if constexpr (E <= 64) {  // density sums are fused on meshes up to 64^3 only
+        if (mode == CHAIN_PLAIN && io.acc[0] && fin) {
+            hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 1, 1>), grid, block, 0, st, io, tw);
+            fused = true;
+        }
+    }
+    if (fused)
+        return;
+    if (mode == CHAIN_PLAIN && fin)
         hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, 1>), grid, block, 0, st, io, tw);
     else if (mode == CHAIN_PLAIN)
         hipLaunchKernelGGL((k_slab_io<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, 0>), grid, block, 0, st, io, tw);
@@ -644,7 +659,7 @@ void launch_one(int mode, const SlabIO& io, const double2* tw, hipStream_t st) {
 template <int E, int R1, int R2>
 void dispatch(int v, const Rep* reps, int nact, int i, int mode, const SlabIO* io, const double2* tw,
               hipStream_t st) {
-    if constexpr (E == 128) {  // one shape only (see dispatch_edge)
+    if constexpr (E >= 96) {  // one shape only (see dispatch_edge)
         (void)v;
         io ? launch_one<E, R1, R2, 1, 1>(mode, *io, tw, st) : launch_set<E, R1, R2, 1, 1>(reps, nact, i, tw, st);
     } else {
@@ -659,11 +674,14 @@ void dispatch(int v, const Rep* reps, int nact, int i, int mode, const SlabIO* i
 
 hipError_t dispatch_edge(int n, int v, const Rep* reps, int nact, int i, int mode, const SlabIO* io,
                          const double2* tw, hipStream_t st) {
-    // 128-point lines: 16 elements per thread leave no registers for the tables, and a
-    // plane of one field fills most of the LDS, so one shape (LDS tables, one field)
+    // 96- and 128-point lines: 12 or 16 elements per thread at 8 waves per workgroup leave
+    // no registers for the tables (the register-table shape spills), and two planes do not
+    // fit in LDS, so one shape (LDS tables, one field per workgroup)
     if (n == 128) dispatch<128, 16, 8>(1, reps, nact, i, mode, io, tw, st);
     else if (n == 64) dispatch<64, 8, 8>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 96) dispatch<96, 12, 8>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 40) dispatch<40, 5, 8>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 16) dispatch<16, 4, 4>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 48) dispatch<48, 12, 4>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 32) dispatch<32, 8, 4>(v, reps, nact, i, mode, io, tw, st);
     else return hipErrorInvalidValue;
@@ -692,6 +710,7 @@ hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* t
     if (one && nact == 1) {
         if (i > one->N - 1) return hipSuccess;
         const SlabIO io = plain_io(*one, i, fuse);
+        if (io.acc[0] && n > 64) return hipErrorInvalidValue;  // no fused kernel above 64^3
         return dispatch_edge(n, variant, nullptr, 1, i, CHAIN_PLAIN, &io, tw, st);
     }
     if (fuse) return hipErrorInvalidValue;  // fused sums run only in the one-replica kernel
@@ -700,6 +719,7 @@ hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* t
 
 hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st, int variant) {
     if (!io_aligned(io) || (io.M & 1)) return hipErrorInvalidValue;
+    if (io.acc[0] && (n > 64 || mode != CHAIN_PLAIN)) return hipErrorInvalidValue;  // no fused kernel
     return dispatch_edge(n, variant, nullptr, 1, 0, mode, &io, tw, st);
 }
 

@@ -129,8 +129,8 @@ __host__ __device__ inline const double* rep_pair(const Rep& R, int j) {
 // i0+1..N ascending, field 1 beads N-i0..1 descending, each split by block A/B) instead
 // of storing their pairs; only pairs 1..i0 and N are stored. For odd N the middle bead
 // i0 pairs two fields of one pair and is added at the end.
-// (Meshes up to 64^3: at 128^3 the passes fill the chip and are bandwidth-bound, so the
-// sums gain nothing there, and their registers would spill.)
+// (Meshes up to 64^3: at 96^3 and 128^3 the passes fill the chip and are bandwidth-bound,
+// so the sums gain nothing there, and their registers would spill.)
 __host__ __device__ inline bool fused_phi(const Rep& R, int fuse) {
     return fuse && R.phacc && R.n && R.n <= 64 && R.N >= 3;
 }
@@ -182,8 +182,8 @@ __host__ __device__ inline SlabIO plain_io(const Rep& R, int i, int fuse = 0) {
 // given, the descriptor travels in the kernel arguments instead.
 bool slab_supported(int n0, int n1, int n2);
 // the cubic edges the slab pass serves, largest first (one launch group each in a set)
-constexpr int SLAB_EDGES[] = {128, 64, 48, 40, 32};
-constexpr int N_SLAB_EDGES = 5;
+constexpr int SLAB_EDGES[] = {128, 96, 64, 48, 40, 32, 16};
+constexpr int N_SLAB_EDGES = 7;
 // `variant` selects one of SLAB_VARIANTS kernel shapes (bit 0: twiddle/kernel tables in
 // LDS instead of registers; bit 1: both fields of a plane in one workgroup). All variants
 // give bit-identical results; the fastest one per mesh size and set size is picked by

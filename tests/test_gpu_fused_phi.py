@@ -21,6 +21,7 @@ CASES = [
     (48, 13, 12),
     (64, 16, 9),
     (40, 13, 7),
+    (16, 9, 4),
     (32, 3, 1),
     (32, 4, 2),
     (32, 7, 0),

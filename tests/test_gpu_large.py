@@ -88,10 +88,11 @@ def test_large_case_langevin_step_both_paths(gpu, monkeypatch):
 
 
 def test_set_with_a_128_member_equals_members_alone(gpu):
-    """A replica set spanning all five slab launch groups (128^3, 64^3, 48^3, 40^3, 32^3)
-    plus a line-path member gives every member's Q and fields bit for bit as it gets alone."""
+    """A replica set spanning all seven slab launch groups (128^3, 96^3, 64^3, 48^3, 40^3,
+    32^3, 16^3) plus a line-path member gives every member's Q and fields bit for bit as it
+    gets alone."""
     specs = [((128,) * 3, 6, 2), ((64,) * 3, 8, 3), ((48,) * 3, 7, 4), ((40,) * 3, 8, 5), ((32,) * 3, 9, 5),
-             ((20, 18, 16), 6, 3)]
+             ((20, 18, 16), 6, 3), ((96,) * 3, 5, 2), ((16,) * 3, 12, 6)]
 
     def make():
         cs = []
@@ -103,7 +104,7 @@ def test_set_with_a_128_member_equals_members_alone(gpu):
             cs.append(c)
         return cs
 
-    print("set of 5 with a 128^3 member", flush=True)
+    print("set of 8 with 128^3 and 96^3 members", flush=True)
     ctxs = make()
     b = L.Batch(ctxs)
     lnq = b.calc_concs()
