@@ -1,0 +1,121 @@
+"""GPU runs of the reference's own inputs against the committed golden fixtures (no oracle
+at run time).
+
+The run is replayed through the C ABI the way src/lfts_simulation.cc:8-135 runs it:
+seed 123456789, M normals for the first noise (src/langevin.cc:16), M uniforms for the
+random W- start (src/lfts_simulation.cc:148-154), the initial Anderson mix, then per step
+the Langevin update, Anderson mix to 1e-4, calc_concs and, in the statistics loop, the
+S(k) sample.
+
+Bars (FP64): Anderson counts equal; lnQ and H to 1e-10 relative; fields to 1e-8 relative
+(max-norm per field, or per field sum, sum of squares and 64 sampled values); S(k) to
+1e-8. (w+ is the Anderson solve's output: its ill-conditioned history solve carries the
+last-bit differences of the evaluations into w+ at about 1e-9 after the 35 iterations of
+the 64^3 initial mix, while lnQ stays within 1e-10.) The `lfts-gpu`
+driver on the 16^3 input prints the fixture's lnQ and H to their 6 printed digits and
+writes the final fields to the 6 digits of its text files."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import lfts_cpu_amd as L
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _header(name):
+    tok = open(os.path.join(GOLD, name + ".input")).read().split()
+    N, NA = int(tok[0]), int(tok[1])
+    XN, C, dt, isXeN = float(tok[2]), float(tok[3]), float(tok[4]), int(tok[5])
+    m = tuple(int(v) for v in tok[6:9])
+    Lb = tuple(float(v) for v in tok[9:12])
+    loop = [int(v) for v in tok[12:17]]
+    return N, NA, XN, C, dt, isXeN, m, Lb, loop
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _replay(name):
+    g = dict(np.load(os.path.join(GOLD, name + ".npz")))
+    N, NA, XN, C, dt, isXeN, m, Lb, (n_eq, n_st, n_smpl, _, _) = _header(name)
+    M = int(np.prod(m))
+    XbN = XN / L.z_inf(Lb, m, N, C * C) if isXeN else XN
+    sigma = math.sqrt(2.0 * M * dt / (C * Lb[0] * Lb[1] * Lb[2]))
+    n_eq, n_st = (n_eq // n_smpl) * n_smpl, (n_st // n_smpl) * n_smpl
+    print(f"golden replay {name}: mesh {m}, N={N}, {n_eq}+{n_st} steps", flush=True)
+    ctx = L.Context(N, NA, m, Lb)
+    rng = L.Rng(123456789, sigma)
+    ctx.langevin_init(rng.normals(M))
+    w = np.zeros(2 * M)
+    w[:M] = XbN * (rng.uniforms(M) - 0.5)
+    ctx.set_fields(w)
+    k0, _ = ctx.anderson_mix(200, 1e-4)
+    w0 = ctx.get_fields(4)
+    lnq, nam = [], []
+    for it in range(1, n_eq + n_st + 1):
+        ctx.langevin_step(rng.normals(M), XbN, dt)
+        k, _ = ctx.anderson_mix(200, 1e-4)
+        lnq.append(ctx.calc_concs())
+        nam.append(k)
+        if it > n_eq and (it - n_eq) % n_smpl == 0:
+            ctx.sk_sample()
+    w_end = ctx.get_fields(4)
+    S, ns = ctx.sk_get()
+    H = ctx.hamiltonian(XbN)
+    ctx.close()
+    return g, M, k0, w0, lnq, nam, w_end, S, ns, H
+
+
+def _check_summary(g, tag, w, M):
+    f = w.reshape(4, M)
+    for k in range(4):
+        s, q = math.fsum(f[k].tolist()), math.fsum((f[k] * f[k]).tolist())
+        ds = abs(s - g[tag + "sums"][k]) / max(abs(g[tag + "sums"][k]), M ** 0.5)
+        dq = abs(q - g[tag + "sumsq"][k]) / g[tag + "sumsq"][k]
+        dp = rel(f[k, g[tag + "points"]], g[tag + "values"][k])
+        print(f"{tag}field {k}: sum {ds:.2e}, sum of squares {dq:.2e}, points {dp:.2e}", flush=True)
+        assert ds < 1e-8 and dq < 1e-8 and dp < 1e-8, (tag, k, ds, dq, dp)
+
+
+@pytest.mark.parametrize("name", ["valgrind16", "input64"])
+def test_reference_input_run_matches_fixture(gpu, name):
+    g, M, k0, w0, lnq, nam, w_end, S, ns, H = _replay(name)
+    assert k0 == int(g["nAM_init"])
+    assert nam == g["nAM"].tolist()
+    assert np.allclose(lnq, g["lnQ"], rtol=1e-10, atol=0), (lnq, g["lnQ"])
+    assert math.isclose(H, float(g["H"]), rel_tol=1e-10)
+    _check_summary(g, "init_", w0, M)
+    if "w_end" in g:
+        for k in range(4):
+            assert rel(w_end[k * M:(k + 1) * M], g["w_end"][k * M:(k + 1) * M]) < 1e-8, k
+        assert ns == int(g["ns"])
+        assert rel(S, g["sk"]) < 1e-8
+    else:
+        _check_summary(g, "end_", w_end, M)
+    print(f"{name}: n_AM {k0} {nam}, lnQ {lnq}, H {H!r}", flush=True)
+
+
+def test_driver_on_reference_test_input(gpu, tmp_path):
+    g = dict(np.load(os.path.join(GOLD, "valgrind16.npz")))
+    inp = os.path.join(GOLD, "valgrind16.input")
+    r = subprocess.run([L.DRIVER_PATH, inp], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    printed = [float(ln[6:]) for ln in r.stdout.splitlines() if ln.startswith("lnQ = ")]
+    assert len(printed) == len(g["lnQ"])
+    for a, b in zip(printed, g["lnQ"]):
+        assert abs(a - b) <= 6e-6 * abs(b), (a, b)
+    H = float(r.stdout.strip().splitlines()[-1])
+    assert abs(H - float(g["H"])) <= 6e-6 * abs(float(g["H"]))
+    M = int(g["M"])
+    vals = np.array([float(x) for x in open(tmp_path / "w_st_1").read().split()[17:]])
+    assert vals.size == 2 * M
+    assert np.allclose(vals, g["w_end"][:2 * M], rtol=6e-6, atol=1e-12)
+    phi = np.array([float(x) for x in open(tmp_path / "phi_st_1").read().split()])
+    assert np.allclose(phi, g["w_end"][2 * M:], rtol=6e-6, atol=1e-12)
