@@ -308,7 +308,7 @@ int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep*
     int best = 0;
     const char* env = std::getenv("LFTS_SLAB_VARIANT");
     char line[160];
-    if (edge >= 96) {  // one shape exists at these sizes (LDS tables, one field per workgroup)
+    if (edge >= 72) {  // one shape exists at these sizes (LDS tables, one field per workgroup)
         best = 1;
         std::snprintf(line, sizeof line, "edge %d x %d: variant 1 (the only shape)\n", edge, cls);
     } else if (env && *env >= '0' && *env < '0' + lfts::SLAB_VARIANTS && env[1] == 0) {

@@ -1,5 +1,6 @@
-// Fused contour step for cubic meshes of 16^3, 32^3, 40^3, 48^3, 64^3, 96^3 and 128^3
-// points: one launch per step over every active replica of that mesh size.
+// Fused contour step for cubic meshes of 16^3, 24^3, 32^3, 40^3, 48^3, 56^3, 64^3, 72^3,
+// 80^3, 96^3, 112^3, 120^3 and 128^3 points: one launch per step over every active replica
+// of that mesh size.
 //
 // The contour step q_{i+1} = h ⊙ C q_i (src/step.cc:35-65) uses the 3D convolution
 // C = Cx Cy Cz, separable into 1D circular convolutions along x, y and z. A slab of the
@@ -25,10 +26,10 @@
 // holding x[j + R2 k]. Forward: radix-R1 DFT in registers, twiddle w_n^{jk'}, exchange
 // through LDS, radix-R2 DFTs -> X[k' + R1 f]; multiply by g; the inverse retraces the
 // steps with conjugate roots. The kernel factor g includes the 1/n of each inverse.
-// Factorisations: 16 = 4 x 4, 32 = 8 x 4, 40 = 5 x 8 (five stage-2 DFTs over eight threads:
-// three idle in that stage), 48 = 12 x 4, 64 = 8 x 8, 96 = 12 x 8 (twelve stage-2 DFTs over
-// eight threads; one plane fills 87 KB of LDS), 128 = 16 x 8 (one plane fills 149 KB of the
-// 160 KB LDS). 96 and 128 have the LDS-table, one-field shape only.
+// Factorisations: 16 = 4 x 4, 32 = 8 x 4, 48 = 12 x 4, and R1 x 8 for 24, 40, 56, 64, 72, 80,
+// 96, 112, 120, 128 (R1 = 3, 5, 7, 8, 9, 10, 12, 14, 15, 16). With R1 < 8 some threads idle
+// in the stage-2 DFTs; with R1 > 8 some take two. One plane of 128 points fills 149 KB of
+// the 160 KB LDS. Edges from 72 up have the LDS-table, one-field shape only.
 // In LDS a line keeps element k*R2 + j at k*(R2+1) + j: the R2 threads of a line then
 // read their contiguous runs from distinct banks.
 //
@@ -275,6 +276,118 @@ __device__ __forceinline__ void dft16(double2* v) {
     }
 }
 
+// natural-order radix-7 DFT: with a_n = x_n + x_(7-n), b_n = x_n - x_(7-n) (n = 1, 2, 3),
+// X_k = x0 + sum_n cos(2 pi n k/7) a_n -/+ i sum_n sin(2 pi n k/7) b_n for X_k / X_(7-k)
+// (forward; the inverse swaps the signs of the i terms)
+template <bool INV>
+__device__ __forceinline__ void dft7(double2* v) {
+    const double c1 = 0.623489801858733530525, c2 = -0.222520933956314404289, c3 = -0.900968867902419126236;
+    const double s1 = 0.781831482468029808708, s2 = 0.974927912181823607018, s3 = 0.433883739117558120476;
+    const double2 x0 = v[0];
+    const double2 a1 = c_add(v[1], v[6]), a2 = c_add(v[2], v[5]), a3 = c_add(v[3], v[4]);
+    const double2 b1 = c_sub(v[1], v[6]), b2 = c_sub(v[2], v[5]), b3 = c_sub(v[3], v[4]);
+    // cos(2 pi n k/7) and sin(2 pi n k/7) for k = 1, 2, 3 and n = 1, 2, 3
+    const double ck[3][3] = {{c1, c2, c3}, {c2, c3, c1}, {c3, c1, c2}};
+    const double sk[3][3] = {{s1, s2, s3}, {s2, -s3, -s1}, {s3, -s1, s2}};
+    v[0] = c_add(x0, c_add(c_add(a1, a2), a3));
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double2 t = make_double2(fma(ck[k][2], a3.x, fma(ck[k][1], a2.x, fma(ck[k][0], a1.x, x0.x))),
+                                       fma(ck[k][2], a3.y, fma(ck[k][1], a2.y, fma(ck[k][0], a1.y, x0.y))));
+        const double2 u = make_double2(fma(sk[k][2], b3.x, fma(sk[k][1], b2.x, sk[k][0] * b1.x)),
+                                       fma(sk[k][2], b3.y, fma(sk[k][1], b2.y, sk[k][0] * b1.y)));
+        const double2 tm = make_double2(t.x + u.y, t.y - u.x), tp = make_double2(t.x - u.y, t.y + u.x);
+        v[k + 1] = INV ? tp : tm;  // t - i u forward
+        v[6 - k] = INV ? tm : tp;
+    }
+}
+
+// cos and sin of 2 pi e / R (0 <= e < R) for the twiddles of the composite lengths; e is a
+// compile-time constant wherever these are used (unrolled loops), so they fold to literals
+template <int R>
+__device__ __forceinline__ double2 root_cs(int e) {
+    if constexpr (R == 9) {
+        constexpr double c[9] = {
+            1.0, 0.766044443118978035202, 0.173648177666930348852, -0.5,
+            -0.939692620785908384054, -0.939692620785908384054, -0.5, 0.173648177666930348852,
+            0.766044443118978035202};
+        constexpr double s[9] = {
+            0.0, 0.642787609686539326323, 0.984807753012208059367, 0.866025403784438646764,
+            0.342020143325668733044, -0.342020143325668733044, -0.866025403784438646764, -0.984807753012208059367,
+            -0.642787609686539326323};
+        return make_double2(c[e], s[e]);
+    } else if constexpr (R == 10) {
+        constexpr double c[10] = {
+            1.0, 0.809016994374947424102, 0.309016994374947424102, -0.309016994374947424102,
+            -0.809016994374947424102, -1.0, -0.809016994374947424102, -0.309016994374947424102,
+            0.309016994374947424102, 0.809016994374947424102};
+        constexpr double s[10] = {
+            0.0, 0.587785252292473129169, 0.951056516295153572116, 0.951056516295153572116,
+            0.587785252292473129169, 0.0, -0.587785252292473129169, -0.951056516295153572116,
+            -0.951056516295153572116, -0.587785252292473129169};
+        return make_double2(c[e], s[e]);
+    } else if constexpr (R == 14) {
+        constexpr double c[14] = {
+            1.0, 0.900968867902419126236, 0.623489801858733530525, 0.222520933956314404289,
+            -0.222520933956314404289, -0.623489801858733530525, -0.900968867902419126236, -1.0,
+            -0.900968867902419126236, -0.623489801858733530525, -0.222520933956314404289, 0.222520933956314404289,
+            0.623489801858733530525, 0.900968867902419126236};
+        constexpr double s[14] = {
+            0.0, 0.433883739117558120476, 0.781831482468029808708, 0.974927912181823607018,
+            0.974927912181823607018, 0.781831482468029808708, 0.433883739117558120476, 0.0,
+            -0.433883739117558120476, -0.781831482468029808708, -0.974927912181823607018, -0.974927912181823607018,
+            -0.781831482468029808708, -0.433883739117558120476};
+        return make_double2(c[e], s[e]);
+    } else if constexpr (R == 15) {
+        constexpr double c[15] = {
+            1.0, 0.913545457642600895502, 0.669130606358858213826, 0.309016994374947424102,
+            -0.1045284632676534714, -0.5, -0.809016994374947424102, -0.978147600733805637929,
+            -0.978147600733805637929, -0.809016994374947424102, -0.5, -0.1045284632676534714,
+            0.309016994374947424102, 0.669130606358858213826, 0.913545457642600895502};
+        constexpr double s[15] = {
+            0.0, 0.406736643075800207754, 0.743144825477394235015, 0.951056516295153572116,
+            0.994521895368273336923, 0.866025403784438646764, 0.587785252292473129169, 0.207911690817759337102,
+            -0.207911690817759337102, -0.587785252292473129169, -0.866025403784438646764, -0.994521895368273336923,
+            -0.951056516295153572116, -0.743144825477394235015, -0.406736643075800207754};
+        return make_double2(c[e], s[e]);
+    } else {
+        static_assert(R == 9 || R == 10 || R == 14 || R == 15, "no twiddle table for this length");
+        return make_double2(1.0, 0.0);
+    }
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(double2* v);
+
+// natural-order DFT of length A B (Cooley-Tukey): x[n2 + B n1] -> length-A DFTs over n1,
+// twiddle w_(AB)^(n2 k1), length-B DFTs over n2 -> X[k1 + A k2]
+template <int A, int B, bool INV>
+__device__ __forceinline__ void dft_ct(double2* v) {
+    double2 y[B][A];
+#pragma unroll
+    for (int n2 = 0; n2 < B; n2++) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) y[n2][n1] = v[n2 + B * n1];
+        dft<A, INV>(y[n2]);
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < B; n2++)
+#pragma unroll
+        for (int k1 = 1; k1 < A; k1++) {
+            const double2 w = root_cs<A * B>(n2 * k1);
+            y[n2][k1] = INV ? c_mul(y[n2][k1], w) : c_mulc(y[n2][k1], w);
+        }
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) {
+        double2 z[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; n2++) z[n2] = y[n2][k1];
+        dft<B, INV>(z);
+#pragma unroll
+        for (int k2 = 0; k2 < B; k2++) v[k1 + A * k2] = z[k2];
+    }
+}
+
 // Inverse DFT of length R of v_n = y_n g_n (g real): the products are folded into the
 // first butterfly stage (pairs n, n + R/2) as fma, then the DFT continues as dft<R, true>.
 template <int R>
@@ -303,11 +416,18 @@ __device__ __forceinline__ void inv_dft_g(double2* y, const double* g) {
 template <int R, bool INV>
 __device__ __forceinline__ void dft(double2* v) {
     if constexpr (R == 16) dft16<INV>(v);
+    else if constexpr (R == 15) dft_ct<3, 5, INV>(v);
+    else if constexpr (R == 14) dft_ct<2, 7, INV>(v);
     else if constexpr (R == 12) dft12<INV>(v);
+    else if constexpr (R == 10) dft_ct<2, 5, INV>(v);
+    else if constexpr (R == 9) dft_ct<3, 3, INV>(v);
     else if constexpr (R == 8) dft8<INV>(v);
+    else if constexpr (R == 7) dft7<INV>(v);
     else if constexpr (R == 5) dft5<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
+    else if constexpr (R == 3) dft3<INV>(v[0], v[1], v[2]);
     else if constexpr (R == 2) dft2<INV>(v[0], v[1]);
+    else static_assert(R == 2, "no register DFT of this length");
 }
 
 // Stage-2 DFTs (of length R2, R1 of them) per thread: up to ceil(R1 / R2). When R2
@@ -659,7 +779,7 @@ void launch_one(int mode, const SlabIO& io, const double2* tw, hipStream_t st) {
 template <int E, int R1, int R2>
 void dispatch(int v, const Rep* reps, int nact, int i, int mode, const SlabIO* io, const double2* tw,
               hipStream_t st) {
-    if constexpr (E >= 96) {  // one shape only (see dispatch_edge)
+    if constexpr (E >= 72) {  // one shape only (see dispatch_edge)
         (void)v;
         io ? launch_one<E, R1, R2, 1, 1>(mode, *io, tw, st) : launch_set<E, R1, R2, 1, 1>(reps, nact, i, tw, st);
     } else {
@@ -674,16 +794,23 @@ void dispatch(int v, const Rep* reps, int nact, int i, int mode, const SlabIO* i
 
 hipError_t dispatch_edge(int n, int v, const Rep* reps, int nact, int i, int mode, const SlabIO* io,
                          const double2* tw, hipStream_t st) {
-    // 96- and 128-point lines: 12 or 16 elements per thread at 8 waves per workgroup leave
-    // no registers for the tables (the register-table shape spills), and two planes do not
-    // fit in LDS, so one shape (LDS tables, one field per workgroup)
+    // Lines of 72 points and more: 9 to 16 elements per thread leave no registers for the
+    // tables (the register-table shapes spill), and two planes either do not fit in LDS or
+    // cap the registers so that the passes spill, so one shape (LDS tables, one field per
+    // workgroup)
     if (n == 128) dispatch<128, 16, 8>(1, reps, nact, i, mode, io, tw, st);
     else if (n == 64) dispatch<64, 8, 8>(v, reps, nact, i, mode, io, tw, st);
-    else if (n == 96) dispatch<96, 12, 8>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 96) dispatch<96, 12, 8>(1, reps, nact, i, mode, io, tw, st);
     else if (n == 40) dispatch<40, 5, 8>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 16) dispatch<16, 4, 4>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 48) dispatch<48, 12, 4>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 32) dispatch<32, 8, 4>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 80) dispatch<80, 10, 8>(1, reps, nact, i, mode, io, tw, st);
+    else if (n == 72) dispatch<72, 9, 8>(1, reps, nact, i, mode, io, tw, st);
+    else if (n == 56) dispatch<56, 7, 8>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 24) dispatch<24, 3, 8>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 112) dispatch<112, 14, 8>(1, reps, nact, i, mode, io, tw, st);
+    else if (n == 120) dispatch<120, 15, 8>(1, reps, nact, i, mode, io, tw, st);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -182,8 +182,8 @@ __host__ __device__ inline SlabIO plain_io(const Rep& R, int i, int fuse = 0) {
 // given, the descriptor travels in the kernel arguments instead.
 bool slab_supported(int n0, int n1, int n2);
 // the cubic edges the slab pass serves, largest first (one launch group each in a set)
-constexpr int SLAB_EDGES[] = {128, 96, 64, 48, 40, 32, 16};
-constexpr int N_SLAB_EDGES = 7;
+constexpr int SLAB_EDGES[] = {128, 120, 112, 96, 80, 72, 64, 56, 48, 40, 32, 24, 16};
+constexpr int N_SLAB_EDGES = 13;
 // `variant` selects one of SLAB_VARIANTS kernel shapes (bit 0: twiddle/kernel tables in
 // LDS instead of registers; bit 1: both fields of a plane in one workgroup). All variants
 // give bit-identical results; the fastest one per mesh size and set size is picked by

@@ -33,7 +33,7 @@ struct lfts_ctx {
     long M = 0, Mk = 0;
     int N = 0, NA = 0, NB = 0, nhMax = 10;
     lfts_abi::AxisPlan ax[3];
-    bool fused = false;       // cubic 16/32/40/48/64/96/128 mesh: one slab launch per contour step
+    bool fused = false;       // cubic mesh of a slab edge (SLAB_EDGES): one slab launch per contour step
     double* w = nullptr;      // 4M
     double* h = nullptr;      // 2M (pair 1)
     double* q = nullptr;      // (N-1) pairs

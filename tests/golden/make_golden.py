@@ -3,9 +3,9 @@
 Inputs are two parameter files held by the reference, copied unchanged:
   - tests/valgrind/input_valgrind (the reference's own test input): 16^3 mesh, N=90,
     random start (loadType 32), one equilibration and one statistics step;
-  - input_files/input64: 64^3 mesh, N=90, random start (loadType 3), two equilibration
-    steps.
-Both fields are generated from the seeded stream (src/lfts_simulation.cc:23,40-44,145-157),
+  - input_files/input64 and input_files/input128: 64^3 and 128^3 meshes, N=90, random
+    start (loadType 3), two equilibration steps.
+All fields are generated from the seeded stream (src/lfts_simulation.cc:23,40-44,145-157),
 so the three lines of each file fix the whole run.
 
 Expected outputs come from the oracle's restatement of the reference's driver
@@ -13,7 +13,7 @@ Expected outputs come from the oracle's restatement of the reference's driver
   constructor (M normals, M uniforms for W-, initial Anderson mix), then per step the
   Langevin update, Anderson mix to 1e-4, calc_concs (the printed lnQ), S(k) sampling
   every sample_freq statistics steps; H at the end.
-For the 16^3 run the fixture keeps the final fields in full; for the 64^3 run it keeps
+For the 16^3 run the fixture keeps the final fields in full; for the larger runs it keeps
 per-field sums, sums of squares and values at 64 fixed points per field.
 
 The noise-stream fixture holds the first outputs of std::mt19937_64 seeded 123456789,
@@ -21,7 +21,8 @@ normal draws of std::normal_distribution(0, 1) on that engine, and uniform draws
 follow them, as the host library produces them.
 
 Run from the repository root in a container that holds the reference at /root/reference:
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [run names]
+(the 128^3 run takes the oracle about half an hour on 8 cores)
 """
 import json
 import math
@@ -41,6 +42,7 @@ REF = "/root/reference"
 RUNS = {
     "valgrind16": os.path.join(REF, "tests/valgrind/input_valgrind"),
     "input64": os.path.join(REF, "input_files/input64"),
+    "input128": os.path.join(REF, "input_files/input128"),
 }
 FULL_FIELDS = {"valgrind16"}
 N_POINTS = 64
@@ -98,6 +100,59 @@ def run(name: str, src: str) -> None:
     print(f"{name}: n_AM(initial) + 1 = {out['nAM_init']}, H = {H!r}", flush=True)
 
 
+# 100 whole Langevin steps of the CPU-runnable workload: diblock N=64, NA=32, XeN=14.0
+# (isXeN=1), C=50, Ndt=0.5, 32^3 mesh, box 4.0^3, W- and w+ i.i.d. N(0, 0.1^2) from numpy
+# seed 1; 50 equilibration steps, then 50 statistics steps sampling S(k) every step.
+TRAJ_HEADER = "64 32 14.0 50 0.5 1\n32 32 32 4.0 4.0 4.0\n50 50 1 1000 1\n"
+TRAJ_SEED = 1
+TRAJ_POINTS = 16
+
+
+def traj_fields(M: int) -> np.ndarray:
+    return np.random.default_rng(TRAJ_SEED).normal(0.0, 0.1, 2 * M)
+
+
+def traj_run() -> None:
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        inp = os.path.join(td, "input")
+        with open(inp, "w") as f:
+            f.write(TRAJ_HEADER)
+        p = O.read_params(inp)
+        M = p["M"]
+        w0 = traj_fields(M)
+        with open(inp, "a") as f:
+            for v in w0:
+                f.write("%.17g\n" % v)
+        sim = O.Sim(inp)
+        k0 = sim.last_nAM()
+        idx = np.sort(np.random.default_rng(11).choice(M, TRAJ_POINTS, replace=False)).astype(np.int64)
+        nsteps = p["equil_its"] + p["sim_its"]
+        lnq = np.zeros(nsteps)
+        nam = np.zeros(nsteps, dtype=np.int64)
+        sums = np.zeros((nsteps, 4))
+        sumsq = np.zeros((nsteps, 4))
+        vals = np.zeros((nsteps, 4, TRAJ_POINTS))
+        for s in range(nsteps):
+            it = s + 1 - p["equil_its"]
+            lnq[s] = sim.step(it >= 1 and it % p["sample_freq"] == 0)
+            nam[s] = sim.last_nAM()
+            f = sim.w().reshape(4, M)
+            for k in range(4):
+                sums[s, k] = math.fsum(f[k].tolist())
+                sumsq[s, k] = math.fsum((f[k] * f[k]).tolist())
+            vals[s] = f[:, idx]
+            if s % 10 == 9:
+                print(f"traj32: step {s + 1}: lnQ = {lnq[s]!r}, n_AM + 1 = {nam[s]}", flush=True)
+        S, ns = sim.sk()
+        np.savez_compressed(
+            os.path.join(HERE, "traj32.npz"),
+            header=np.array(TRAJ_HEADER), seed=TRAJ_SEED, M=M, XbN=p["chi_b"], sigma=p["sigma"],
+            w0_sums=np.array([math.fsum(w0[:M].tolist()), math.fsum(w0[M:].tolist())]),
+            w0_head=w0[:4], nAM_init=k0, lnQ=lnq, nAM=nam, sums=sums, sumsq=sumsq, points=idx,
+            values=vals, sk=S, ns=ns, H_end=sim.getH())
+
+
 def rng_fixture() -> None:
     raw = O.mt64(123456789, 8)
     z = O.normals(123456789, 1.0, 16)
@@ -120,6 +175,9 @@ def rng_fixture() -> None:
 if __name__ == "__main__":
     O.build()
     rng_fixture()
-    names = sys.argv[1:] or list(RUNS)
+    names = sys.argv[1:] or list(RUNS) + ["traj32"]
     for n in names:
-        run(n, RUNS[n])
+        if n == "traj32":
+            traj_run()
+        else:
+            run(n, RUNS[n])

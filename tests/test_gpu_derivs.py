@@ -15,7 +15,7 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-# line path (odd sizes, radices 2/3/5, both chain-end quirks) and the slab path (16^3, 32^3, 40^3, 48^3, 64^3, 96^3, 128^3)
+# line path (odd sizes, radices 2/3/5, both chain-end quirks) and the slab path (16^3, 32^3, 40^3, 48^3, 56^3, 64^3, 80^3, 96^3, 128^3)
 CASES = [
     ((4, 5, 6), (3.0, 3.5, 4.0), 12, 5),
     ((6, 4, 5), (2.5, 3.0, 2.0), 9, 9),
@@ -28,6 +28,8 @@ CASES = [
     ((128, 128, 128), (6.0, 6.0, 6.0), 4, 2),
     ((96, 96, 96), (5.2, 5.2, 5.2), 5, 2),
     ((16, 16, 16), (3.0, 3.0, 3.0), 11, 4),
+    ((56, 56, 56), (4.6, 4.6, 4.6), 7, 3),
+    ((80, 80, 80), (5.0, 5.0, 5.0), 5, 2),
 ]
 
 

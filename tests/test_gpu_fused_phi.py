@@ -22,6 +22,8 @@ CASES = [
     (64, 16, 9),
     (40, 13, 7),
     (16, 9, 4),
+    (24, 9, 4),
+    (56, 11, 6),
     (32, 3, 1),
     (32, 4, 2),
     (32, 7, 0),

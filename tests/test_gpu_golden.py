@@ -119,3 +119,72 @@ def test_driver_on_reference_test_input(gpu, tmp_path):
     assert np.allclose(vals, g["w_end"][:2 * M], rtol=6e-6, atol=1e-12)
     phi = np.array([float(x) for x in open(tmp_path / "phi_st_1").read().split()])
     assert np.allclose(phi, g["w_end"][2 * M:], rtol=6e-6, atol=1e-12)
+
+
+def test_100_langevin_steps_match_fixture(gpu):
+    """100 whole Langevin steps of the CPU-runnable workload (32^3, N=64, NA=32, XeN 14.0,
+    C=50, Ndt=0.5, Gaussian start from numpy seed 1; 50 equilibration steps, then 50
+    statistics steps sampling S(k) every step). Bars: equal Anderson counts at every
+    step; Q (as lnQ), phi-, phi+ and H within 1e-10 relative after the first step and
+    within 1e-8 at every step of the 100; W-, w+ within 1e-8; S(k) after the 50 samples
+    within 1e-8."""
+    g = dict(np.load(os.path.join(GOLD, "traj32.npz")))
+    tok = str(g["header"]).split()
+    N, NA = int(tok[0]), int(tok[1])
+    XN, C, dt, isXeN = float(tok[2]), float(tok[3]), float(tok[4]), int(tok[5])
+    m = tuple(int(v) for v in tok[6:9])
+    Lb = tuple(float(v) for v in tok[9:12])
+    n_eq, n_st, n_smpl = int(tok[12]), int(tok[13]), int(tok[14])
+    M = int(np.prod(m))
+    XbN = XN / L.z_inf(Lb, m, N, C * C) if isXeN else XN
+    assert math.isclose(XbN, float(g["XbN"]), rel_tol=1e-14)
+    sigma = math.sqrt(2.0 * M * dt / (C * Lb[0] * Lb[1] * Lb[2]))
+    w0 = np.random.default_rng(int(g["seed"])).normal(0.0, 0.1, 2 * M)
+    assert np.array_equal(w0[:4], g["w0_head"])     # the generator still gives the fixture's start
+    assert math.fsum(w0[:M].tolist()) == g["w0_sums"][0] and math.fsum(w0[M:].tolist()) == g["w0_sums"][1]
+    print(f"100-step trajectory: mesh {m}, N={N}, NA={NA}", flush=True)
+    ctx = L.Context(N, NA, m, Lb)
+    rng = L.Rng(123456789, sigma)
+    ctx.langevin_init(rng.normals(M))
+    ctx.set_fields(w0)
+    k0, _ = ctx.anderson_mix(200, 1e-4)
+    assert k0 == int(g["nAM_init"])
+    idx = g["points"]
+    nsteps = n_eq + n_st
+    assert nsteps == 100 == len(g["lnQ"])
+    worst = {"lnQ": 0.0, "H": 0.0, "W-": 0.0, "w+": 0.0, "phi-": 0.0, "phi+": 0.0}
+    names = ["W-", "w+", "phi-", "phi+"]
+    for s in range(nsteps):
+        it = s + 1 - n_eq
+        ctx.langevin_step(rng.normals(M), XbN, dt)
+        k, _ = ctx.anderson_mix(200, 1e-4)
+        lnq = ctx.calc_concs()
+        if it >= 1 and it % n_smpl == 0:
+            ctx.sk_sample()
+        assert k == int(g["nAM"][s]), s
+        f = ctx.get_fields(4).reshape(4, M)
+        err = {"lnQ": abs(lnq - g["lnQ"][s]) / max(1.0, abs(g["lnQ"][s]))}
+        sums = [math.fsum(f[q].tolist()) for q in range(4)]
+        sumsq = [math.fsum((f[q] * f[q]).tolist()) for q in range(4)]
+        H = -lnq + (sumsq[0] / XbN - sums[1]) / M
+        H_ref = -g["lnQ"][s] + (g["sumsq"][s, 0] / XbN - g["sums"][s, 1]) / M
+        err["H"] = abs(H - H_ref) / max(1.0, abs(H_ref))
+        for q in range(4):
+            e_sum = abs(sums[q] - g["sums"][s, q]) / max(abs(g["sums"][s, q]), M ** 0.5)
+            e_sq = abs(sumsq[q] - g["sumsq"][s, q]) / g["sumsq"][s, q]
+            err[names[q]] = max(e_sum, e_sq, rel(f[q, idx], g["values"][s, q]))
+        for key, e in err.items():
+            worst[key] = max(worst[key], e)
+            assert e < 1e-8, (s, key, e)
+        if s == 0:
+            for key in ("lnQ", "H", "phi-", "phi+"):
+                assert err[key] < 1e-10, (key, err[key])
+        if s % 20 == 19:
+            print(f"step {s + 1}: " + ", ".join(f"{a} {b:.1e}" for a, b in err.items()), flush=True)
+    S, ns = ctx.sk_get()
+    assert ns == int(g["ns"]) == n_st // n_smpl
+    e_sk = rel(S, g["sk"])
+    print("worst over 100 steps: " + ", ".join(f"{a} {b:.1e}" for a, b in worst.items()) + f"; S(k) {e_sk:.1e}",
+          flush=True)
+    assert e_sk < 1e-8
+    ctx.close()

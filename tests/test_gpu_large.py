@@ -88,11 +88,13 @@ def test_large_case_langevin_step_both_paths(gpu, monkeypatch):
 
 
 def test_set_with_a_128_member_equals_members_alone(gpu):
-    """A replica set spanning all seven slab launch groups (128^3, 96^3, 64^3, 48^3, 40^3,
-    32^3, 16^3) plus a line-path member gives every member's Q and fields bit for bit as it
-    gets alone."""
+    """A replica set spanning all thirteen slab launch groups (every cubic slab edge from
+    16^3 to 128^3) plus a line-path member gives every member's Q and fields bit for bit as
+    it gets alone."""
     specs = [((128,) * 3, 6, 2), ((64,) * 3, 8, 3), ((48,) * 3, 7, 4), ((40,) * 3, 8, 5), ((32,) * 3, 9, 5),
-             ((20, 18, 16), 6, 3), ((96,) * 3, 5, 2), ((16,) * 3, 12, 6)]
+             ((20, 18, 16), 6, 3), ((96,) * 3, 5, 2), ((16,) * 3, 12, 6), ((24,) * 3, 7, 3),
+             ((56,) * 3, 6, 2), ((72,) * 3, 5, 3), ((80,) * 3, 5, 2), ((112,) * 3, 4, 2),
+             ((120,) * 3, 4, 1)]
 
     def make():
         cs = []
@@ -104,7 +106,7 @@ def test_set_with_a_128_member_equals_members_alone(gpu):
             cs.append(c)
         return cs
 
-    print("set of 8 with 128^3 and 96^3 members", flush=True)
+    print("set of 14 with a member of every slab edge", flush=True)
     ctxs = make()
     b = L.Batch(ctxs)
     lnq = b.calc_concs()

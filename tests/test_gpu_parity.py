@@ -203,9 +203,10 @@ def test_langevin_trajectory(gpu, tmp_path, m, N, NA, XbN, steps):
 
 
 @pytest.mark.parametrize("n,N,NA", [(64, 12, 5), (32, 20, 13), (64, 9, 9), (48, 10, 4), (40, 11, 6), (128, 6, 2),
-                                      (96, 7, 3), (16, 10, 10), (16, 5, 0)])
+                                      (96, 7, 3), (16, 10, 10), (16, 5, 0), (24, 9, 4), (56, 8, 3),
+                                      (72, 7, 7), (80, 6, 2), (112, 5, 3), (120, 5, 1)])
 def test_fused_slab_path_against_oracle_and_generic_path(gpu, monkeypatch, n, N, NA):
-    """The one-launch-per-step slab path (cubic 16^3/32^3/40^3/48^3/64^3/96^3/128^3) against the oracle
+    """The one-launch-per-step slab path (every cubic slab edge from 16^3 to 128^3) against the oracle
     and the three-pass line path on the same input."""
     m, Lb = (n, n, n), (4.4, 4.4, 4.4)
     M = n ** 3

@@ -15,7 +15,7 @@ def _fields(M, seed):
     return np.concatenate([rng.normal(0, 2.0, M), rng.normal(0, 1.0, M)])
 
 
-@pytest.mark.parametrize("n", [16, 32, 40, 48, 64])
+@pytest.mark.parametrize("n", [16, 24, 32, 40, 48, 56, 64])
 def test_every_shape_gives_the_same_bits(gpu, monkeypatch, n):
     m, N, NA = (n, n, n), 12, 5
     M = n ** 3
