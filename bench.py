@@ -33,7 +33,7 @@ METRIC = "Langevin steps/s at 64³,N=90 FP64 (plain vs batched); % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
 NK_STEPS = 10  # Langevin steps timed with the Newton-Krylov solver in the derivative leg
 HOST_STEPS = 5  # Langevin steps with host noise (PCIe-inclusive rate)
-PROF_STEPS = 3  # steps after the timed region with live event timing of the contour chains
+PROF_EVERY = 8  # live event timing of every 8th propagator evaluation (a timed one costs ~10 us)
 
 
 def log(*a):
@@ -147,6 +147,8 @@ def batched_leg(steps, warmup, L_mod, torch, quick=False):
         B.full_step_dev([e.data_ptr() for e in etas[s]], XbN, 0.5)
     # snapshot the states so that the sequential leg starts from the same point
     snap = [(c.get_fields(4), None) for c in ctxs]
+    # every set evaluation timed: its chain takes milliseconds, so the events' cost is
+    # small, and the bytes of a sampled evaluation would depend on which members are active
     B.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -331,12 +333,17 @@ def main():
     log(f"initial mix: k={k0} err={err0:.3e} ({time.perf_counter() - t0:.2f} s)")
 
     nsteps = args.warmup + args.steps
-    n_extra = PROF_STEPS + (0 if args.no_adjoint else 1 + NK_STEPS)
+    n_extra = 0 if args.no_adjoint else 1 + NK_STEPS
     eta = torch.empty((nsteps + n_extra, M), dtype=torch.float64, device="cuda")
     for s in range(nsteps + n_extra):
         eta[s].copy_(torch.from_numpy(rng.normals(M)))
     torch.cuda.synchronize()
 
+    # live HIP-event timing of the contour chains on the context's stream (the roofline
+    # figure) over the timed steps, on every PROF_EVERY-th propagator evaluation. Already on
+    # during the warm-up, so that the timed steps reuse pooled events; enabling it again
+    # before the timed steps zeroes the sums.
+    ctx.profile(True, PROF_EVERY)
     for s in range(args.warmup):
         lnq, k = ctx.full_step_dev(eta[s].data_ptr(), XbN, dt)
         log(f"warmup {s}: lnQ={lnq:.10g} n_AM={k - 1}")
@@ -344,6 +351,7 @@ def main():
 
     nAM = []
     marks = []
+    ctx.profile(True, PROF_EVERY)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.warmup, nsteps):
@@ -354,11 +362,6 @@ def main():
     elapsed = time.perf_counter() - t0
     step_ms = np.diff(np.array([t0] + marks)) * 1e3
     log(f"timed {args.steps} steps: {elapsed:.3f} s, n_AM={nAM}, lnQ={lnq:.10g}")
-    # a few more steps with live HIP-event timing of the contour chains (the roofline
-    # figure), outside the timed region
-    ctx.profile(True)
-    for s in range(nsteps, nsteps + PROF_STEPS):
-        ctx.full_step_dev(eta[s].data_ptr(), XbN, dt)
     ms_chain, n_cs, ms_pe, n_pe = ctx.profile_read()
     ctx.profile(False)
     tuning_plain = ctx.tuning().strip()
@@ -380,7 +383,7 @@ def main():
 
     adjoint = None
     if not args.no_adjoint:
-        e0 = nsteps + PROF_STEPS
+        e0 = nsteps
         adjoint = adjoint_leg(ctx, eta[e0].data_ptr(), [eta[e0 + 1 + k] for k in range(NK_STEPS)], XbN, dt, M, torch)
         log(f"adjoint leg: {adjoint}")
 
@@ -445,7 +448,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": measured_traffic(args.mesh),
             "kernel": "contour step (48*M algorithmic bytes; plain and fused-density-sum passes, 44:45 per "
-                      f"evaluation), HIP events on the context stream over {PROF_STEPS} further steps",
+                      f"evaluation), HIP events on the context stream over every {PROF_EVERY}th evaluation of the timed steps",
             "contour_step_us": cs_ms * 1e3,
             "attainable_copy_GBps": copy_gbps,
             "slab_tuning": tuning_plain,

@@ -127,12 +127,14 @@ int lfts_step_fwd(lfts_ctx* ctx, const double* q_in, double* q_out, const double
 int lfts_counters(lfts_ctx* ctx, long long* n_prop_evals, long long* n_contour_steps);
 
 /* Live timing with HIP events recorded on the context stream (no reference counterpart;
- * the reference only prints whole-second wall times, src/fts_cpu.cc:44-57). While enabled,
- * every propagator evaluation records the duration of its contour-step chain (the N-1
- * contour-step launches, back to back) and of the whole evaluation. lfts_profile_read
- * returns the sums (ms) and counts (contour-step launches, evaluations) since the last
- * enable. For a replica set, one contour-step launch advances every member of one mesh
- * size, and evaluations are counted per member. */
+ * the reference only prints whole-second wall times, src/fts_cpu.cc:44-57). With enable =
+ * k >= 1, every k-th propagator evaluation (the first, the (k+1)-th, ...) records the
+ * duration of its contour-step chain (the N-1 contour-step launches, back to back) and of
+ * the whole evaluation; 0 turns the timing off. A recorded evaluation costs about 10 us of
+ * host and queue time outside its chain, so k > 1 keeps that cost off the others.
+ * lfts_profile_read returns the sums (ms) and counts (contour-step launches, evaluations)
+ * of the recorded evaluations since the last enable. For a replica set, one contour-step
+ * launch advances every member of one mesh size, and evaluations are counted per member. */
 int lfts_profile(lfts_ctx* ctx, int enable);
 int lfts_profile_read(lfts_ctx* ctx, double* ms_chain, long long* n_contour_steps, double* ms_pe,
                       long long* n_prop_evals);

@@ -318,8 +318,9 @@ class Context:
         return lnQ.value, {"newton": info.newton, "krylov": info.krylov, "prop_evals": info.prop_evals,
                            "err": info.err}
 
-    def profile(self, enable: bool = True):
-        self._chk(lib().lfts_profile(self.h, int(enable)))
+    def profile(self, enable: bool = True, every: int = 1):
+        """Live event timing of every `every`-th propagator evaluation (off with enable=False)."""
+        self._chk(lib().lfts_profile(self.h, int(every) if enable else 0))
 
     def profile_read(self):
         """(ms in contour-step chains, contour steps, ms in whole evaluations, evaluations)"""
@@ -409,8 +410,9 @@ class Batch:
     def tuning(self) -> str:
         return lib().lfts_batch_tuning(self.h).decode()
 
-    def profile(self, enable: bool = True):
-        self._chk(lib().lfts_batch_profile(self.h, int(enable)))
+    def profile(self, enable: bool = True, every: int = 1):
+        """Live event timing of every `every`-th evaluation of the set (off with enable=False)."""
+        self._chk(lib().lfts_batch_profile(self.h, int(every) if enable else 0))
 
     def profile_read(self):
         a, c = ctypes.c_double(), ctypes.c_double()

@@ -359,13 +359,15 @@ int calc_concs_async(lfts_batch* b, bool store_all) {
     // same bytes either way and keeps the leaner kernel.
     const int fuse = (!store_all && nact == 1) ? 1 : 0;
     lfts_batch::EvQuad ev{};
-    if (b->prof) {
+    // live timing of every prof-th evaluation (its events cost host and queue time)
+    const bool rec = b->prof > 0 && (b->prof_seq++ % b->prof) == 0;
+    if (rec) {
         for (hipEvent_t* e : {&ev.pe0, &ev.ch0, &ev.ch1, &ev.pe1})
             if (int rc = get_event(b, e)) return rc;
         BHIPCHK(b, hipEventRecord(ev.pe0, b->st));
     }
     BHIPCHK(b, lfts::launch_prepare_h(b->d_ract, nact, b->nb_max, b->st));
-    if (b->prof) BHIPCHK(b, hipEventRecord(ev.ch0, b->st));
+    if (rec) BHIPCHK(b, hipEventRecord(ev.ch0, b->st));
     long long npasses = 0;
     // slab passes: i = 0 prologue, i = 1..N-1 contour steps (src/diblock.cc:43). With
     // more than one mesh size each group's chain runs on its own stream.
@@ -407,11 +409,11 @@ int calc_concs_async(lfts_batch* b, bool store_all) {
             BHIPCHK(b, hipEventRecord(b->ev_join[g], b->gst[g]));
             BHIPCHK(b, hipStreamWaitEvent(b->st, b->ev_join[g], 0));
         }
-    if (b->prof) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
+    if (rec) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
     BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_part, b->st));
     BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, 1, 1, b->d_scal, 0, b->st));
     BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, fuse, b->st));
-    if (b->prof) {
+    if (rec) {
         BHIPCHK(b, hipEventRecord(ev.pe1, b->st));
         ev.npasses = npasses;
         ev.nrep = nact;
@@ -1140,7 +1142,9 @@ int lfts_batch_full_step(lfts_batch* b, const double* const* eta_new, const doub
 int lfts_batch_profile(lfts_batch* b, int enable) {
     if (!b) return LFTS_EINVAL;
     if (int rc = sync_batch(b)) return rc;
-    b->prof = enable != 0;
+    if (enable < 0) return bfail(b, LFTS_EINVAL, "profile: enable must be >= 0");
+    b->prof = enable;
+    b->prof_seq = 0;
     b->prof_ms_chain = b->prof_ms_pe = 0.0;
     b->prof_n_pass = b->prof_n_pe = 0;
     return LFTS_OK;

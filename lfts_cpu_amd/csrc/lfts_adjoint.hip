@@ -907,7 +907,6 @@ int lfts_box_derivative(lfts_ctx* c, double* dHdL, double* lnQ) {
     if (!c || !dHdL) return fail(c, LFTS_EINVAL, "box_derivative: null argument");
     if (c->NA < 1 || c->NA > c->N - 1)
         return fail(c, LFTS_EINVAL, "box_derivative: needs 1 <= NA <= N-1 (the q+ chain is then the q chain's adjoint)");
-    if (c->ck_stride) return fail(c, LFTS_EINVAL, "box_derivative: needs every propagator pair stored");
     if (int rc = ensure_deriv(c)) return rc;
     hipStream_t st = c->self->st;
     const long M = c->M;
@@ -917,10 +916,11 @@ int lfts_box_derivative(lfts_ctx* c, double* dHdL, double* lnQ) {
     const int nb = grid_for(M);
     hipLaunchKernelGGL(lfts::k_fill, dim3(nb), dim3(256), 0, st, c->acc, M, 0.0);
     HIPCHK(c, hipGetLastError());
-    for (int i = 1; i < c->N; i++) {
+    // bead i: one complex transform of q_i + i q+_{i+1}, then the |k|-weighted products
+    auto bead = [&](int, const double* qi, const double* qd) -> int {
         LineArgs z = line_args(c, 2, lfts::LINE_FFT);
-        z.inA = pair_ptr(c, i);
-        z.inB = pair_ptr(c, c->N - i) + M;  // q+_{i+1}
+        z.inA = qi;
+        z.inB = qd;
         z.outC = c->zk;
         HIPCHK(c, lfts::launch_line(z, st));
         LineArgs y = line_args(c, 1, lfts::LINE_FFT);
@@ -934,6 +934,13 @@ int lfts_box_derivative(lfts_ctx* c, double* dHdL, double* lnQ) {
         hipLaunchKernelGGL(lfts::k_stress_acc, dim3(nb), dim3(256), 0, st, (const double2*)c->zk, c->acc,
                            c->d.m[0], c->d.m[1], c->d.m[2]);
         HIPCHK(c, hipGetLastError());
+        return LFTS_OK;
+    };
+    if (c->ck_stride) {  // pairs recomputed segment by segment from the checkpoints
+        if (int rc = ckpt_for_bead_pairs(c, bead)) return rc;
+    } else {
+        for (int i = 1; i < c->N; i++)
+            if (int rc = bead(i, pair_ptr(c, i), pair_ptr(c, c->N - i) + M)) return rc;  // q+_{i+1}
     }
     lfts::StressArgs a{};
     a.acc = c->acc;

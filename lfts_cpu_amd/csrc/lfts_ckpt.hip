@@ -385,6 +385,35 @@ int ckpt_adj_finish(lfts_ctx* c, const double* ubar, double* wbar) {
 
 const double* ckpt_pair_N(lfts_ctx* c, bool derived) { return ck_slot(c, derived ? c->ck_d : c->ck_f, c->N); }
 
+// Beads i = 1..N-1 with q_i and q+_{i+1} (the box derivative's products), after a
+// checkpointed evaluation. q_i is field 0 of pair i and q+_{i+1} field 1 of pair N - i: one
+// place off the mirror N + 1 - i, so segment s against its mirror covers i = b_s ..
+// b_{s+1} - 2, and the segment's last bead takes q+ from the next segment's mirror, with
+// q_i carried over. Every segment is recomputed twice (as itself and as a mirror).
+int ckpt_for_bead_pairs(lfts_ctx* c, const std::function<int(int, const double*, const double*)>& fn) {
+    const long M = c->M;
+    const int N = c->N, Kc = K(c);
+    double* X = SEG(c, 0);
+    double* carry = RING(c, 0);  // q_i of the previous segment's last bead
+    for (int s = 0; s < Kc; s++) {
+        const int sm = Kc - 1 - s;
+        double* Y = (sm == s) ? X : SEG(c, 1);
+        if (int rc = recompute_fwd(c, s, X)) return rc;
+        if (sm != s)
+            if (int rc = recompute_fwd(c, sm, Y)) return rc;
+        const int b0 = bnd(c, s), b1 = bnd(c, s + 1) - 1, y0 = bnd(c, sm);
+        auto qd = [&](int i) { return Y + 2L * ((N - i) - y0) * M + M; };  // q+_{i+1}
+        if (s > 0)
+            if (int rc = fn(b0 - 1, carry, qd(b0 - 1))) return rc;
+        for (int i = b0; i < b1; i++)
+            if (int rc = fn(i, X + 2L * (i - b0) * M, qd(i))) return rc;
+        if (b1 < N) HIPCHK(c, hipMemcpyAsync(carry, X + 2L * (b1 - b0) * M, sizeof(double) * M,
+                                             hipMemcpyDeviceToDevice, c->self->st));
+    }
+    c->n_cs += 2L * (N - 1);
+    return LFTS_OK;
+}
+
 // stride 0: every pair stored; stride S >= 2: pairs kept at about N/S symmetric segment
 // boundaries (the derivative chains' storage shrinks the same way).
 int set_pair_storage(lfts_ctx* c, int stride) {

@@ -110,7 +110,8 @@ struct lfts_batch {
     std::string tune_report;
     hipEvent_t tune_ev[2] = {nullptr, nullptr};
     // live event timing
-    bool prof = false;
+    int prof = 0;            // live timing: every prof-th propagator evaluation (0: off)
+    long long prof_seq = 0;  // evaluations since the timing was enabled
     std::vector<hipEvent_t> ev_pool;
     struct EvQuad { hipEvent_t pe0, ch0, ch1, pe1; long long npasses; int nrep; };
     std::vector<EvQuad> ev_pending;
@@ -177,6 +178,7 @@ int ckpt_derived(lfts_ctx* c, int mode, const double* d1, const double* dstart1)
 int ckpt_tan_finish(lfts_ctx* c, const double* e, const double* dQ, double* dphi);
 int ckpt_adj_finish(lfts_ctx* c, const double* ubar, double* wbar);
 const double* ckpt_pair_N(lfts_ctx* c, bool derived);
+int ckpt_for_bead_pairs(lfts_ctx* c, const std::function<int(int, const double*, const double*)>& fn);
 int set_pair_storage(lfts_ctx* c, int stride);
 void release_pair_storage(lfts_ctx* c);
 int dif_pos(int k, const std::vector<int>& rad, size_t s, int B);

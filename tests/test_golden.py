@@ -97,3 +97,31 @@ def test_oracle_reproduces_reference_test_input_run():
     assert ns == int(g["ns"]) == 1
     assert rel(S, g["sk"]) < 1e-12
     assert math.isclose(sim.getH(), float(g["H"]), rel_tol=1e-12)
+
+
+def test_trajectory_fixture_start_and_first_steps(tmp_path):
+    """The 100-step trajectory fixture (32^3, N=64): numpy still generates its starting
+    fields, and the oracle reproduces its first three steps (Anderson counts exactly; lnQ,
+    field sums and sampled values to 1e-12 relative)."""
+    g = _load("traj32")
+    header = str(g["header"])
+    M = int(g["M"])
+    w0 = np.random.default_rng(int(g["seed"])).normal(0.0, 0.1, 2 * M)
+    assert np.array_equal(w0[:4], g["w0_head"])
+    assert math.fsum(w0[:M].tolist()) == g["w0_sums"][0] and math.fsum(w0[M:].tolist()) == g["w0_sums"][1]
+    f = tmp_path / "input"
+    with open(f, "w") as fh:
+        fh.write(header)
+        for v in w0:
+            fh.write("%.17g\n" % v)
+    sim = O.Sim(str(f))
+    assert sim.last_nAM() == int(g["nAM_init"])
+    idx = g["points"]
+    for s in range(3):
+        lnq = sim.step(False)
+        assert sim.last_nAM() == int(g["nAM"][s])
+        assert abs(lnq - g["lnQ"][s]) <= 1e-12 * max(1.0, abs(g["lnQ"][s]))
+        w = sim.w().reshape(4, M)
+        for k in range(4):
+            assert abs(math.fsum(w[k].tolist()) - g["sums"][s, k]) <= 1e-12 * max(abs(g["sums"][s, k]), M ** 0.5)
+            assert rel(w[k, idx], g["values"][s, k]) < 1e-12

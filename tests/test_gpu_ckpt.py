@@ -1,7 +1,7 @@
 """Checkpointed propagator storage (pairs kept at symmetric segment starts, segments
 recomputed with their mirrors) against full storage: the same propagator evaluation,
-Anderson solve, tangent and adjoint products to 1e-12 relative (only the order of the
-bead sums differs), on the slab and line paths, with even and odd N, and the chain-end
+Anderson solve, tangent and adjoint products and box derivative to 1e-12 relative (only
+the order of the bead sums differs), on the slab and line paths, with even and odd N, and the chain-end
 factors of NA = 0 and NA = N."""
 import numpy as np
 import pytest
@@ -53,6 +53,9 @@ def test_checkpointed_matches_full_storage(gpu, m, Lb, N, NA, S):
     assert rel(b.jvp(u, v), a.jvp(u, v)) < 1e-12
     assert rel(b.vjp(ub), a.vjp(ub)) < 1e-12
     if 1 <= NA <= N - 1:
+        (da, qa), (db, qb) = a.box_derivative(), b.box_derivative()
+        assert qa == qb or abs(qa - qb) <= 1e-13 * max(1.0, abs(qa))
+        assert rel(np.array(db), np.array(da)) < 1e-12
         ka, ea = a.anderson_mix(200, 1e-6)
         kb, eb = b.anderson_mix(200, 1e-6)
         assert ka == kb
@@ -76,6 +79,7 @@ def test_full_size_checkpointed(gpu):
     assert rel(b.get_fields(4)[2 * M:], a.get_fields(4)[2 * M:]) < 1e-12
     assert rel(b.jvp(None, v), a.jvp(None, v)) < 1e-12
     assert rel(b.vjp(ub), a.vjp(ub)) < 1e-12
+    assert rel(np.array(b.box_derivative()[0]), np.array(a.box_derivative()[0])) < 1e-12
     ia = a.saddle_nk(tol=1e-9)
     ib = b.saddle_nk(tol=1e-9)
     assert ib["err"] <= 1e-9
@@ -85,15 +89,15 @@ def test_full_size_checkpointed(gpu):
 def test_checkpointed_limits(gpu):
     c = L.Context(12, 6, (8, 8, 8), (3.0, 3.0, 3.0))
     c.set_pair_storage(3)
-    c.set_fields(np.zeros(2 * 512))
+    c.set_fields(_fields(512, 4))
     c.calc_concs()
     with pytest.raises(L.LftsError):
         c.pair(4)
     with pytest.raises(L.LftsError):
-        c.box_derivative()
-    with pytest.raises(L.LftsError):
         L.Batch([c])
+    d_ck, _ = c.box_derivative()
     c.set_pair_storage(0)  # back to full storage
     assert c.pair_storage() == (0, 12)
     c.calc_concs()
     c.pair(4)
+    assert rel(d_ck, c.box_derivative()[0]) < 1e-12
