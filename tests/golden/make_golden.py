@@ -1,10 +1,12 @@
 """Writes the golden fixtures in this directory (data only: inputs and expected outputs).
 
-Inputs are two parameter files held by the reference, copied unchanged:
+Inputs are three parameter files held by the reference, copied unchanged:
   - tests/valgrind/input_valgrind (the reference's own test input): 16^3 mesh, N=90,
     random start (loadType 32), one equilibration and one statistics step;
   - input_files/input64 and input_files/input128: 64^3 and 128^3 meshes, N=90, random
-    start (loadType 3), two equilibration steps.
+    start (loadType 3), two equilibration steps;
+and one written here, example40: the reference's example parameters (input_files/input,
+40^3) with a random start (loadType 3) and ten equilibration steps.
 All fields are generated from the seeded stream (src/lfts_simulation.cc:23,40-44,145-157),
 so the three lines of each file fix the whole run.
 
@@ -43,6 +45,10 @@ RUNS = {
     "valgrind16": os.path.join(REF, "tests/valgrind/input_valgrind"),
     "input64": os.path.join(REF, "input_files/input64"),
     "input128": os.path.join(REF, "input_files/input128"),
+    # the parameters of the reference's example input (input_files/input: N=90, NA=40,
+    # XbN=17.148912, C=100, Ndt=1, 40^3, box 4.38^3) with the random start of loadType 3 in
+    # place of its 1.6 MB field listing, and ten equilibration steps
+    "example40": "90 40 17.148912 100.000000 1.000000 0\n40 40 40 4.38 4.38 4.38\n10 0 10 1000 3\n",
 }
 FULL_FIELDS = {"valgrind16"}
 N_POINTS = 64
@@ -65,7 +71,11 @@ def field_summary(w: np.ndarray, M: int) -> dict:
 
 def run(name: str, src: str) -> None:
     inp = os.path.join(HERE, name + ".input")
-    shutil.copyfile(src, inp)
+    if src.startswith("/"):
+        shutil.copyfile(src, inp)
+    else:
+        with open(inp, "w") as f:
+            f.write(src)
     sim = O.Sim(inp)
     p = sim.params
     M = p["M"]

@@ -102,6 +102,21 @@ def test_reference_input_run_matches_fixture(gpu, name):
     print(f"{name}: n_AM {k0} {nam}, lnQ {lnq}, H {H!r}", flush=True)
 
 
+def test_driver_on_reference_input64(gpu, tmp_path):
+    """The `lfts-gpu` driver on the reference's 64^3 input (random start, initial mix, two
+    equilibration steps): its printed lnQ and H equal the fixture's to their 6 digits."""
+    g = dict(np.load(os.path.join(GOLD, "input64.npz")))
+    r = subprocess.run([L.DRIVER_PATH, os.path.join(GOLD, "input64.input")], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    printed = [float(ln[6:]) for ln in r.stdout.splitlines() if ln.startswith("lnQ = ")]
+    assert len(printed) == len(g["lnQ"]) == 2
+    for a, b in zip(printed, g["lnQ"]):
+        assert abs(a - b) <= 6e-6 * abs(b), (a, b)
+    H = float(r.stdout.strip().splitlines()[-1])
+    assert abs(H - float(g["H"])) <= 6e-6 * abs(float(g["H"]))
+
+
 def test_driver_on_reference_test_input(gpu, tmp_path):
     g = dict(np.load(os.path.join(GOLD, "valgrind16.npz")))
     inp = os.path.join(GOLD, "valgrind16.input")
