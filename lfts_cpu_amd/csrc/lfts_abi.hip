@@ -715,8 +715,11 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     *out = nullptr;
     if (desc->N < 2 || desc->NA < 0 || desc->NA > desc->N) return LFTS_EINVAL;
     if (desc->nhMax < 1 || desc->nhMax > lfts::MAX_HIST) return LFTS_EINVAL;
-    for (int d = 0; d < 3; d++)
+    for (int d = 0; d < 3; d++) {
         if (desc->m[d] < 1 || !(desc->L[d] > 0.0)) return LFTS_EINVAL;
+        std::vector<int> rad;  // the line transforms take factors 2, 3, 5 and 7 only
+        if (!factorize(desc->m[d], rad) || (int)rad.size() > lfts::MAXST) return LFTS_EINVAL;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LFTS_EHIP;
     lfts_ctx* c = new lfts_ctx();

@@ -105,6 +105,10 @@ def test_create_rejects_bad_descriptors():
         L.Context(10, 5, (4, 4, 4), (1.0, -1.0, 1.0))
     with pytest.raises(L.LftsError):
         L.Context(10, 5, (4, 4, 4), (1.0, 1.0, 1.0), nhMax=0)
+    for m in ((22, 4, 4), (4, 13, 4), (4, 4, 0)):  # a prime factor above 7, an empty axis
+        with pytest.raises(L.LftsError) as e:
+            L.Context(10, 5, m, (1.0, 1.0, 1.0))
+        assert e.value.code == L.LFTS_EINVAL
 
 
 @pytest.mark.skipif(has_gpu(), reason="checks the no-device error path")
