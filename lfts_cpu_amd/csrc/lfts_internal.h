@@ -187,7 +187,8 @@ constexpr int N_SLAB_EDGES = 13;
 // `variant` selects one of SLAB_VARIANTS kernel shapes (bit 0: twiddle/kernel tables in
 // LDS instead of registers; bit 1: both fields of a plane in one workgroup). All variants
 // give bit-identical results; the fastest one per mesh size and set size is picked by
-// timing (the autotuner of the replica sets).
+// timing (the autotuner of the replica sets). Edges from 72 up have variant 1 only (the
+// other shapes spill registers or exceed the LDS).
 constexpr int SLAB_VARIANTS = 4;
 // fuse = 1: density sums fused into the second half of the chain (fused_phi); only for
 // one replica (nact == 1 with its host descriptor `one`).
