@@ -84,7 +84,7 @@ def _check_summary(g, tag, w, M):
         assert ds < 1e-8 and dq < 1e-8 and dp < 1e-8, (tag, k, ds, dq, dp)
 
 
-@pytest.mark.parametrize("name", ["valgrind16", "input64"])
+@pytest.mark.parametrize("name", ["valgrind16", "input64", "example40"])
 def test_reference_input_run_matches_fixture(gpu, name):
     g, M, k0, w0, lnq, nam, w_end, S, ns, H = _replay(name)
     assert k0 == int(g["nAM_init"])
