@@ -103,6 +103,29 @@ def test_anderson_mix(gpu, m, Lb, N, NA):
     assert math.isclose(err, am.last_err(), rel_tol=1e-6)
 
 
+@pytest.mark.parametrize("m,nhMax", [((8, 6, 10), 1), ((8, 6, 10), 3), ((32, 32, 32), 7), ((8, 6, 10), 15)])
+def test_anderson_history_depths(gpu, m, nhMax):
+    """anderson(M, maxHist) with other history depths (src/anderson.h:30; the reference
+    driver uses 10): the ring wraps after nhMax slots, with equal iteration counts and
+    w+, phi to 1e-9 against the oracle; a 32^3 case runs the slab path."""
+    M = int(np.prod(m))
+    Lb, N, NA = (3.1, 2.8, 3.4), 14, 6
+    w = _fields(M, 40 + nhMax, 3.0, 1.0)
+    ctx = L.Context(N, NA, m, Lb, nhMax=nhMax)
+    ctx.set_fields(w[:2 * M])
+    k, err = ctx.anderson_mix(200, 1e-7)
+    got = ctx.get_fields(4)
+    d = O.Diblock(N, NA, m, Lb)
+    am = O.Anderson(M, nhMax)
+    wo = w.copy()
+    k_ref = am.mix(d, 200, 1e-7, wo)
+    print(f"nhMax {nhMax}: k = {k}", flush=True)
+    assert k == k_ref and k > nhMax + 2  # the history ring wraps at least once
+    assert rel(got[M:2 * M], wo[M:2 * M]) < 1e-9
+    assert rel(got[2 * M:], wo[2 * M:]) < 1e-9
+    assert math.isclose(err, am.last_err(), rel_tol=1e-6)
+
+
 def test_langevin_step_host_and_device_noise(gpu):
     import torch
 
