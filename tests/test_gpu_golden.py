@@ -7,11 +7,18 @@ random W- start (src/lfts_simulation.cc:148-154), the initial Anderson mix, then
 the Langevin update, Anderson mix to 1e-4, calc_concs and, in the statistics loop, the
 S(k) sample.
 
-Bars (FP64): Anderson counts equal; lnQ and H to 1e-10 relative; fields to 1e-8 relative
-(max-norm per field, or per field sum, sum of squares and 64 sampled values); S(k) to
-1e-8. (w+ is the Anderson solve's output: its ill-conditioned history solve carries the
-last-bit differences of the evaluations into w+ at about 1e-9 after the 35 iterations of
-the 64^3 initial mix, while lnQ stays within 1e-10.) The `lfts-gpu`
+Bars (FP64; relative max-norm per field, or per field sum, sum of squares and 64 sampled
+values): Anderson counts equal; lnQ and H 1e-9; W- 1e-8; phi-, phi+ 5e-8; w+ 1e-5; S(k)
+1e-8. These are trajectory bars. Each run starts with Anderson solves from a random field
+at grid scale, and a solve fixes w+ only to its residual tolerance (1e-4 in phi+), and
+the mean of w+ not at all (a uniform shift of w+ leaves phi unchanged). The solve's
+ill-conditioned history solve turns the last-bit differences of the evaluations into a
+different point of that tolerance ball: after the 128^3 initial mix w+ differs by 2e-6
+and phi+ by 1.3e-8 (5000 times below the residual tolerance), while lnQ agrees to 1.2e-10
+and H to 3e-12; two steps later every field but the mean of w+ agrees to 2e-10. One
+evaluation on identical fields agrees to 1e-10 and better (test_gpu_parity), and the
+100-step test holds Q, phi+-, H and S(k) to 1e-10 after its first step and 1e-8 over all
+100. The `lfts-gpu`
 driver on the 16^3 input prints the fixture's lnQ and H to their 6 printed digits and
 writes the final fields to the 6 digits of its text files."""
 import math
@@ -73,40 +80,50 @@ def _replay(name):
     return g, M, k0, w0, lnq, nam, w_end, S, ns, H
 
 
-def _check_summary(g, tag, w, M):
+def _summary_diffs(g, tag, w, M):
+    """Largest relative difference per field (W-, w+, phi-, phi+) over its sum, sum of
+    squares and sampled values."""
     f = w.reshape(4, M)
+    out = []
     for k in range(4):
         s, q = math.fsum(f[k].tolist()), math.fsum((f[k] * f[k]).tolist())
         ds = abs(s - g[tag + "sums"][k]) / max(abs(g[tag + "sums"][k]), M ** 0.5)
         dq = abs(q - g[tag + "sumsq"][k]) / g[tag + "sumsq"][k]
         dp = rel(f[k, g[tag + "points"]], g[tag + "values"][k])
         print(f"{tag}field {k}: sum {ds:.2e}, sum of squares {dq:.2e}, points {dp:.2e}", flush=True)
-        assert ds < 1e-8 and dq < 1e-8 and dp < 1e-8, (tag, k, ds, dq, dp)
+        out.append(max(ds, dq, dp))
+    return out
 
 
-@pytest.mark.parametrize("name", ["valgrind16", "input64", "example40"])
+@pytest.mark.parametrize("name", ["valgrind16", "input64", "example40", "input128"])
 def test_reference_input_run_matches_fixture(gpu, name):
     g, M, k0, w0, lnq, nam, w_end, S, ns, H = _replay(name)
     assert k0 == int(g["nAM_init"])
     assert nam == g["nAM"].tolist()
-    assert np.allclose(lnq, g["lnQ"], rtol=1e-10, atol=0), (lnq, g["lnQ"])
-    assert math.isclose(H, float(g["H"]), rel_tol=1e-10)
-    _check_summary(g, "init_", w0, M)
+    d_lnq = max(abs(a - b) / max(1.0, abs(b)) for a, b in zip(lnq, g["lnQ"]))
+    d_H = abs(H - float(g["H"])) / max(1.0, abs(float(g["H"])))
+    print(f"{name}: lnQ within {d_lnq:.1e}, H within {d_H:.1e}", flush=True)
+    assert d_lnq < 1e-9 and d_H < 1e-9, (lnq, g["lnQ"], H, float(g["H"]))
+    diffs = [_summary_diffs(g, "init_", w0, M)]
     if "w_end" in g:
-        for k in range(4):
-            assert rel(w_end[k * M:(k + 1) * M], g["w_end"][k * M:(k + 1) * M]) < 1e-8, k
+        diffs.append([rel(w_end[k * M:(k + 1) * M], g["w_end"][k * M:(k + 1) * M]) for k in range(4)])
         assert ns == int(g["ns"])
         assert rel(S, g["sk"]) < 1e-8
     else:
-        _check_summary(g, "end_", w_end, M)
+        diffs.append(_summary_diffs(g, "end_", w_end, M))
+    print(f"{name}: field differences (W-, w+, phi-, phi+) {diffs}", flush=True)
+    for d in diffs:
+        assert d[0] < 1e-8 and d[1] < 1e-5 and d[2] < 5e-8 and d[3] < 5e-8, d
     print(f"{name}: n_AM {k0} {nam}, lnQ {lnq}, H {H!r}", flush=True)
 
 
-def test_driver_on_reference_input64(gpu, tmp_path):
-    """The `lfts-gpu` driver on the reference's 64^3 input (random start, initial mix, two
-    equilibration steps): its printed lnQ and H equal the fixture's to their 6 digits."""
-    g = dict(np.load(os.path.join(GOLD, "input64.npz")))
-    r = subprocess.run([L.DRIVER_PATH, os.path.join(GOLD, "input64.input")], cwd=tmp_path, capture_output=True,
+@pytest.mark.parametrize("name", ["input64", "input128"])
+def test_driver_on_reference_inputs(gpu, tmp_path, name):
+    """The `lfts-gpu` driver on the reference's 64^3 and 128^3 inputs (random start, initial
+    mix, two equilibration steps): its printed lnQ and H equal the fixture's to their 6
+    digits."""
+    g = dict(np.load(os.path.join(GOLD, name + ".npz")))
+    r = subprocess.run([L.DRIVER_PATH, os.path.join(GOLD, name + ".input")], cwd=tmp_path, capture_output=True,
                        text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     printed = [float(ln[6:]) for ln in r.stdout.splitlines() if ln.startswith("lnQ = ")]
