@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mesh", type=int, default=64)
+    ap.add_argument("--dims", type=int, nargs=3, default=None,
+                    help="a non-cubic mesh mx my mz instead of --mesh (same workload otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batched-steps", type=int, default=3, help="timed steps of the replica-set leg (0 skips it)")
     ap.add_argument("--batched-quick", action="store_true", help="8 members instead of 64 (smoke runs)")
@@ -316,7 +318,7 @@ def main():
         return
 
     N, NA, XbN, C, dt = 90, 45, 17.0, 50.0, 0.5
-    m = (args.mesh,) * 3
+    m = tuple(args.dims) if args.dims else (args.mesh,) * 3
     Lb = (4.4, 4.4, 4.4)
     M = int(np.prod(m))
     V = Lb[0] * Lb[1] * Lb[2]
@@ -446,7 +448,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": measured_traffic(args.mesh),
+            "traffic": None if args.dims else measured_traffic(args.mesh),
             "kernel": "contour step (48*M algorithmic bytes; plain and fused-density-sum passes, 44:45 per "
                       f"evaluation), HIP events on the context stream over every {PROF_EVERY}th evaluation of the timed steps",
             "contour_step_us": cs_ms * 1e3,

@@ -133,6 +133,8 @@ LineArgs line_args(const lfts_ctx* c, int axis, int mode) {
     a.tw = p.tw;
     a.gperm = p.gperm;
     a.perm = p.perm;
+    a.gnat = p.gnat;
+    a.fast = c->fast_lines ? 1 : 0;
     a.axis = axis;
     a.m0 = c->d.m[0];
     a.m1 = c->d.m[1];
@@ -743,11 +745,13 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
 #define CREATE_HIP(expr) CREATE_CHK((expr) == hipSuccess ? LFTS_OK : LFTS_EHIP)
     for (int d = 0; d < 3; d++) CREATE_CHK(build_axis(c, c->ax[d], desc->m[d], desc->L[d], c->N));
     {
-        // LFTS_PATH=generic selects the three-pass line kernels even where the fused
-        // slab pass applies (used to cross-check the two paths)
+        // LFTS_PATH=generic selects the three-pass generic line kernels (runtime radix
+        // stages in LDS) even where the fused slab pass or the register-FFT axis kernel
+        // applies (used to cross-check the paths)
         const char* path = std::getenv("LFTS_PATH");
         const bool generic = path && std::strcmp(path, "generic") == 0;
         c->fused = !generic && lfts::slab_supported(desc->m[0], desc->m[1], desc->m[2]);
+        c->fast_lines = !generic;  // line passes of slab-edge lengths use register FFTs
     }
     CREATE_HIP(hipMalloc(&c->w, sizeof(double) * 4 * M));
     CREATE_HIP(hipMalloc(&c->h, sizeof(double) * 2 * M));

@@ -850,4 +850,115 @@ hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, 
     return dispatch_edge(n, variant, nullptr, 1, 0, mode, &io, tw, st);
 }
 
+// ---- convolution along one axis with register FFTs (line path) ---------------------------
+// For meshes the slab pass does not serve, a contour step is three passes, one per axis
+// (src/step.cc:35-65 as Cz, Cy, Cx; the last multiplies by h). A pass whose line length is
+// one of the slab edges runs here: 32 complex lines per workgroup, each line the two real
+// fields of a pair at one position (field A + i field B), held by R2 threads with R1
+// elements each and convolved by conv_line with the tables in LDS. For x- and y-lines the
+// lanes of a wave take consecutive lines (adjacent in memory), so each load or store
+// covers a 256-byte run; for z-lines (contiguous) the R2 threads of a line are adjacent.
+// The finish follows the generic kernel's rules (plain, times h, tangent, adjoint).
+namespace {
+
+constexpr int AX_LINES = 32;
+
+__device__ __forceinline__ long axis_line_base(const LineArgs& a, long line) {
+    if (a.axis == 2) return line * a.m2;
+    if (a.axis == 1) {
+        const long x = line / a.m2, z = line - x * a.m2;
+        return x * (long)a.m1 * a.m2 + z;
+    }
+    return line;  // axis 0: line = y m2 + z
+}
+
+}  // namespace
+
+template <int E, int R1, int R2, bool ZAX>
+__global__ __launch_bounds__(AX_LINES * R2) void k_axis(const LineArgs a) {
+    static_assert(R1 * R2 == E, "unsupported factorisation");
+    constexpr int XP = (R1 * (R2 + 1)) | 1;  // exchange row pitch (complex), odd
+    constexpr int XBYTES = AX_LINES * XP * 16;
+    __shared__ __align__(16) unsigned char smem[XBYTES + E * 16 + E * 8];
+    if (a.skip && *a.skip) return;
+    const int t = threadIdx.x;
+    const int c = ZAX ? t / R2 : t % AX_LINES;  // line within the workgroup
+    const int j = ZAX ? t % R2 : t / AX_LINES;  // lane within the line
+    const long line = (long)blockIdx.x * AX_LINES + c;
+    const bool live = line < a.nlines;
+    const long base = live ? axis_line_base(a, line) : 0;
+    const long S = a.stride;
+    double2 z[R1];
+#pragma unroll
+    for (int k = 0; k < R1; k++) {
+        const long p = base + (long)(j + R2 * k) * S;
+        z[k] = live ? make_double2(a.inA[p], a.inB[p]) : make_double2(0.0, 0.0);
+    }
+    double2* stw = reinterpret_cast<double2*>(smem + XBYTES);
+    double* sg = reinterpret_cast<double*>(smem + XBYTES + E * 16);
+    for (int k = t; k < E; k += AX_LINES * R2) {
+        stw[k] = a.tw[k];
+        sg[k] = a.gnat[k];
+    }
+    lds_barrier();
+    LdsTab<R1, R2> tab;
+    tab.load(stw, sg, j);
+    conv_line<R1, R2>(z, reinterpret_cast<double2*>(smem) + c * XP, j, tab);
+    if (!live) return;
+#pragma unroll
+    for (int k = 0; k < R1; k++) {
+        const long p = base + (long)(j + R2 * k) * S;
+        double x = z[k].x, y = z[k].y;
+        if (a.hA) {
+            if (a.chain == CHAIN_TANGENT) {
+                x = fma(x, a.hA[p], a.auxA[p] * a.srcA[p]);
+                y = fma(y, a.hB[p], a.auxB[p] * a.srcB[p]);
+            } else if (a.chain == CHAIN_ADJOINT) {
+                x = fma(a.auxA[p], a.srcA[p], x);
+                y = fma(a.auxB[p], a.srcB[p], y);
+                a.out2A[p] = x * a.hA[p];
+                a.out2B[p] = y * a.hB[p];
+            } else {
+                x *= a.hA[p];
+                y *= a.hB[p];
+            }
+        }
+        a.outA[p] = x;
+        a.outB[p] = y;
+    }
+}
+
+namespace {
+
+template <int E, int R1, int R2>
+hipError_t axis_launch(const LineArgs& a, hipStream_t st) {
+    const unsigned nblk = (unsigned)((a.nlines + AX_LINES - 1) / AX_LINES);
+    if (a.axis == 2) hipLaunchKernelGGL((k_axis<E, R1, R2, true>), dim3(nblk), dim3(AX_LINES * R2), 0, st, a);
+    else hipLaunchKernelGGL((k_axis<E, R1, R2, false>), dim3(nblk), dim3(AX_LINES * R2), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
+    if (a.mode != LINE_CONV || !a.gnat || !a.inA || !a.inB || !a.outA || !a.outB) return hipErrorNotSupported;
+    switch (a.n) {
+        case 16: return axis_launch<16, 4, 4>(a, st);
+        case 24: return axis_launch<24, 3, 8>(a, st);
+        case 32: return axis_launch<32, 8, 4>(a, st);
+        case 40: return axis_launch<40, 5, 8>(a, st);
+        case 48: return axis_launch<48, 12, 4>(a, st);
+        case 56: return axis_launch<56, 7, 8>(a, st);
+        case 64: return axis_launch<64, 8, 8>(a, st);
+        case 72: return axis_launch<72, 9, 8>(a, st);
+        case 80: return axis_launch<80, 10, 8>(a, st);
+        case 96: return axis_launch<96, 12, 8>(a, st);
+        case 112: return axis_launch<112, 14, 8>(a, st);
+        case 120: return axis_launch<120, 15, 8>(a, st);
+        case 128: return axis_launch<128, 16, 8>(a, st);
+        default: return hipErrorNotSupported;
+    }
+}
+
+
 }  // namespace lfts

@@ -91,8 +91,15 @@ struct LineArgs {
     double* outR;
     const double* mul;
     const int* skip;      // device flag: when set, the launch returns at once (or null)
+    // LINE_CONV on a line length the slab pass serves: fast = 1 runs the register-FFT
+    // axis kernel (natural-order kernel values gnat, g(k)/n) instead of the generic one
+    const double* gnat;
+    int fast;
 };
 hipError_t launch_line(const LineArgs& a, hipStream_t st);
+// the register-FFT convolution along one axis (lfts_fused.hip); hipErrorNotSupported
+// when the line length is not one of SLAB_EDGES
+hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st);
 
 // ---- fused slab pass over replicas with a cubic mesh of edge n -----------------------
 

@@ -239,6 +239,10 @@ __global__ __launch_bounds__(256) void k_line(LineArgs a) {
 }
 
 hipError_t launch_line(const LineArgs& a, hipStream_t st) {
+    if (a.mode == LINE_CONV && a.fast) {
+        const hipError_t e = launch_axis_conv(a, st);
+        if (e != hipErrorNotSupported) return e;
+    }
     const long nblk = (a.nlines + a.T - 1) / a.T;
     const size_t lds = sizeof(double2) * (size_t)a.T * a.pitch;
     hipLaunchKernelGGL(k_line, dim3((unsigned)nblk), dim3(256), lds, st, a);

@@ -34,6 +34,7 @@ struct lfts_ctx {
     int N = 0, NA = 0, NB = 0, nhMax = 10;
     lfts_abi::AxisPlan ax[3];
     bool fused = false;       // cubic mesh of a slab edge (SLAB_EDGES): one slab launch per contour step
+    bool fast_lines = true;   // line-path convolutions of slab-edge lengths: register-FFT axis kernel
     double* w = nullptr;      // 4M
     double* h = nullptr;      // 2M (pair 1)
     double* q = nullptr;      // (N-1) pairs

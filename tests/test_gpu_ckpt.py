@@ -19,6 +19,7 @@ CASES = [
     ((96, 96, 96), (5.2, 5.2, 5.2), 8, 3, 3),
     ((16, 16, 16), (3.0, 3.0, 3.0), 14, 7, 4),
     ((72, 72, 72), (4.8, 4.8, 4.8), 10, 4, 3),
+    ((24, 40, 16), (3.2, 4.1, 2.6), 11, 5, 3),
     ((6, 5, 8), (2.5, 2.0, 3.5), 13, 13, 3),
     ((8, 6, 5), (3.0, 2.5, 2.0), 12, 0, 5),
     ((10, 12, 7), (3.2, 3.6, 2.9), 21, 9, 4),

@@ -30,6 +30,7 @@ CASES = [
     ((16, 16, 16), (3.0, 3.0, 3.0), 11, 4),
     ((56, 56, 56), (4.6, 4.6, 4.6), 7, 3),
     ((80, 80, 80), (5.0, 5.0, 5.0), 5, 2),
+    ((32, 24, 16), (3.6, 3.1, 2.5), 9, 4),     # line path with register-FFT axis passes
 ]
 
 

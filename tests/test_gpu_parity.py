@@ -21,6 +21,8 @@ MESHES = [
     ((40, 40, 40), (4.38, 4.38, 4.38), 8, 4),  # the reference's 40^3 example mesh
     ((7, 9, 14), (2.1, 2.7, 3.9), 9, 5),     # radix 7 and 9 = 3*3
     ((32, 32, 32), (4.0, 4.0, 4.0), 64, 32),
+    ((32, 16, 24), (3.5, 2.8, 3.1), 10, 4),    # line path, register-FFT axis passes (edge lengths)
+    ((64, 40, 48), (4.1, 3.3, 3.7), 8, 5),
 ]
 
 
@@ -252,6 +254,34 @@ def test_fused_slab_path_against_oracle_and_generic_path(gpu, monkeypatch, n, N,
     for i in (2, N // 2 + 1, N):
         assert rel(ctx.pair(i), d.pair(i)) < 1e-10, i
         assert rel(ctx.pair(i), gen.pair(i)) < 1e-12, i
+
+
+@pytest.mark.parametrize("m,Lb,N,NA", [((32, 16, 24), (3.5, 2.8, 3.1), 10, 4), ((128, 64, 56), (5.0, 3.9, 3.3), 6, 3),
+                                       ((16, 12, 120), (2.5, 2.2, 6.0), 7, 7)])
+def test_axis_kernel_against_generic_line_kernel(gpu, monkeypatch, m, Lb, N, NA):
+    """Line-path meshes whose axes have slab-edge lengths run the register-FFT axis kernel;
+    LFTS_PATH=generic runs the generic line kernel on the same input. Pairs, lnQ and phi
+    agree to 1e-12 relative (different FFT arithmetic), and both to the oracle (the first
+    two meshes, 1e-10)."""
+    M = int(np.prod(m))
+    w = _fields(M, 21, 3.0, 1.0)
+    ctx = L.Context(N, NA, m, Lb)
+    ctx.set_fields(w[:2 * M])
+    lnq = ctx.calc_concs()
+    monkeypatch.setenv("LFTS_PATH", "generic")
+    gen = L.Context(N, NA, m, Lb)
+    gen.set_fields(w[:2 * M])
+    lnq_gen = gen.calc_concs()
+    assert abs(lnq - lnq_gen) <= 1e-12 * max(1.0, abs(lnq_gen))
+    assert rel(ctx.get_fields(4)[2 * M:], gen.get_fields(4)[2 * M:]) < 1e-12
+    for i in (2, N // 2 + 1, N):
+        assert rel(ctx.pair(i), gen.pair(i)) < 1e-12, i
+    if M <= 64 * 40 * 48:
+        d = O.Diblock(N, NA, m, Lb)
+        wo = w.copy()
+        lnq_ref = d.calc_concs(wo)
+        assert abs(lnq - lnq_ref) <= 1e-10 * max(1.0, abs(lnq_ref))
+        assert rel(ctx.get_fields(4)[2 * M:], wo[2 * M:]) < 1e-10
 
 
 @pytest.mark.parametrize("mode", ["direct", "graph"])
