@@ -350,8 +350,70 @@ __device__ __forceinline__ double2 root_cs(int e) {
             -0.207911690817759337102, -0.587785252292473129169, -0.866025403784438646764, -0.994521895368273336923,
             -0.951056516295153572116, -0.743144825477394235015, -0.406736643075800207754};
         return make_double2(c[e], s[e]);
+    } else if constexpr (R == 18) {
+        constexpr double c[18] = {
+            1.0, 0.939692620785908384054, 0.766044443118978035202, 0.5,
+            0.173648177666930348852, -0.173648177666930348852, -0.5, -0.766044443118978035202,
+            -0.939692620785908384054, -1.0, -0.939692620785908384054, -0.766044443118978035202,
+            -0.5, -0.173648177666930348852, 0.173648177666930348852, 0.5,
+            0.766044443118978035202, 0.939692620785908384054};
+        constexpr double s[18] = {
+            0.0, 0.342020143325668733044, 0.642787609686539326323, 0.866025403784438646764,
+            0.984807753012208059367, 0.984807753012208059367, 0.866025403784438646764, 0.642787609686539326323,
+            0.342020143325668733044, 0.0, -0.342020143325668733044, -0.642787609686539326323,
+            -0.866025403784438646764, -0.984807753012208059367, -0.984807753012208059367, -0.866025403784438646764,
+            -0.642787609686539326323, -0.342020143325668733044};
+        return make_double2(c[e], s[e]);
+    } else if constexpr (R == 20) {
+        constexpr double c[20] = {
+            1.0, 0.951056516295153572116, 0.809016994374947424102, 0.587785252292473129169,
+            0.309016994374947424102, 0.0, -0.309016994374947424102, -0.587785252292473129169,
+            -0.809016994374947424102, -0.951056516295153572116, -1.0, -0.951056516295153572116,
+            -0.809016994374947424102, -0.587785252292473129169, -0.309016994374947424102, 0.0,
+            0.309016994374947424102, 0.587785252292473129169, 0.809016994374947424102, 0.951056516295153572116};
+        constexpr double s[20] = {
+            0.0, 0.309016994374947424102, 0.587785252292473129169, 0.809016994374947424102,
+            0.951056516295153572116, 1.0, 0.951056516295153572116, 0.809016994374947424102,
+            0.587785252292473129169, 0.309016994374947424102, 0.0, -0.309016994374947424102,
+            -0.587785252292473129169, -0.809016994374947424102, -0.951056516295153572116, -1.0,
+            -0.951056516295153572116, -0.809016994374947424102, -0.587785252292473129169, -0.309016994374947424102};
+        return make_double2(c[e], s[e]);
+    } else if constexpr (R == 24) {
+        constexpr double c[24] = {
+            1.0, 0.96592582628906828675, 0.866025403784438646764, 0.707106781186547524401,
+            0.5, 0.258819045102520762349, 0.0, -0.258819045102520762349,
+            -0.5, -0.707106781186547524401, -0.866025403784438646764, -0.96592582628906828675,
+            -1.0, -0.96592582628906828675, -0.866025403784438646764, -0.707106781186547524401,
+            -0.5, -0.258819045102520762349, 0.0, 0.258819045102520762349,
+            0.5, 0.707106781186547524401, 0.866025403784438646764, 0.96592582628906828675};
+        constexpr double s[24] = {
+            0.0, 0.258819045102520762349, 0.5, 0.707106781186547524401,
+            0.866025403784438646764, 0.96592582628906828675, 1.0, 0.96592582628906828675,
+            0.866025403784438646764, 0.707106781186547524401, 0.5, 0.258819045102520762349,
+            0.0, -0.258819045102520762349, -0.5, -0.707106781186547524401,
+            -0.866025403784438646764, -0.96592582628906828675, -1.0, -0.96592582628906828675,
+            -0.866025403784438646764, -0.707106781186547524401, -0.5, -0.258819045102520762349};
+        return make_double2(c[e], s[e]);
+    } else if constexpr (R == 25) {
+        constexpr double c[25] = {
+            1.0, 0.96858316112863111949, 0.876306680043863587308, 0.728968627421411523147,
+            0.535826794978996618271, 0.309016994374947424102, 0.0627905195293133760762, -0.187381314585724630543,
+            -0.425779291565072648863, -0.637423989748689710177, -0.809016994374947424102, -0.929776485888251403661,
+            -0.99211470131447783105, -0.99211470131447783105, -0.929776485888251403661, -0.809016994374947424102,
+            -0.637423989748689710177, -0.425779291565072648863, -0.187381314585724630543, 0.0627905195293133760762,
+            0.309016994374947424102, 0.535826794978996618271, 0.728968627421411523147, 0.876306680043863587308,
+            0.96858316112863111949};
+        constexpr double s[25] = {
+            0.0, 0.248689887164854788242, 0.481753674101715274987, 0.684547105928688673732,
+            0.844327925502015078549, 0.951056516295153572116, 0.998026728428271561952, 0.982287250728688681086,
+            0.904827052466019527714, 0.770513242775789230803, 0.587785252292473129169, 0.368124552684677959157,
+            0.125333233564304245373, -0.125333233564304245373, -0.368124552684677959157, -0.587785252292473129169,
+            -0.770513242775789230803, -0.904827052466019527714, -0.982287250728688681086, -0.998026728428271561952,
+            -0.951056516295153572116, -0.844327925502015078549, -0.684547105928688673732, -0.481753674101715274987,
+            -0.248689887164854788242};
+        return make_double2(c[e], s[e]);
     } else {
-        static_assert(R == 9 || R == 10 || R == 14 || R == 15, "no twiddle table for this length");
+        static_assert(R == 9, "no twiddle table for this length");
         return make_double2(1.0, 0.0);
     }
 }
@@ -388,34 +450,44 @@ __device__ __forceinline__ void dft_ct(double2* v) {
     }
 }
 
-// Inverse DFT of length R of v_n = y_n g_n (g real): the products are folded into the
-// first butterfly stage (pairs n, n + R/2) as fma, then the DFT continues as dft<R, true>.
+// Inverse DFT of length R of v_n = y_n g_n (g real). For R = 4 and 8 the products are
+// folded into the first butterfly stage (pairs n, n + R/2) as fma, then the DFT continues
+// as dft<R, true>; other lengths multiply first.
 template <int R>
 __device__ __forceinline__ void inv_dft_g(double2* y, const double* g) {
-    static_assert(R == 4 || R == 8, "kernel-multiply fusion for radix 4 and 8");
-    constexpr int Hh = R / 2;
-    double2 a[Hh], b[Hh];
+    if constexpr (R != 4 && R != 8) {
 #pragma unroll
-    for (int n = 0; n < Hh; n++) {
-        const double2 t = make_double2(y[n + Hh].x * g[n + Hh], y[n + Hh].y * g[n + Hh]);
-        a[n] = make_double2(fma(y[n].x, g[n], t.x), fma(y[n].y, g[n], t.y));
-        b[n] = make_double2(fma(y[n].x, g[n], -t.x), fma(y[n].y, g[n], -t.y));
-    }
-    if constexpr (R == 8) {
-        dft8_tail<true>(a, b, y);
+        for (int n = 0; n < R; n++) y[n] = make_double2(y[n].x * g[n], y[n].y * g[n]);
+        dft<R, true>(y);
     } else {
-        // radix 4: a = (s02, s13), b = (d02, v1 - v3)
-        const double2 d13 = c_rot<true>(b[1]);
-        y[0] = c_add(a[0], a[1]);
-        y[2] = c_sub(a[0], a[1]);
-        y[1] = c_add(b[0], d13);
-        y[3] = c_sub(b[0], d13);
+        constexpr int Hh = R / 2;
+        double2 a[Hh], b[Hh];
+#pragma unroll
+        for (int n = 0; n < Hh; n++) {
+            const double2 t = make_double2(y[n + Hh].x * g[n + Hh], y[n + Hh].y * g[n + Hh]);
+            a[n] = make_double2(fma(y[n].x, g[n], t.x), fma(y[n].y, g[n], t.y));
+            b[n] = make_double2(fma(y[n].x, g[n], -t.x), fma(y[n].y, g[n], -t.y));
+        }
+        if constexpr (R == 8) {
+            dft8_tail<true>(a, b, y);
+        } else {
+            // radix 4: a = (s02, s13), b = (d02, v1 - v3)
+            const double2 d13 = c_rot<true>(b[1]);
+            y[0] = c_add(a[0], a[1]);
+            y[2] = c_sub(a[0], a[1]);
+            y[1] = c_add(b[0], d13);
+            y[3] = c_sub(b[0], d13);
+        }
     }
 }
 
 template <int R, bool INV>
 __device__ __forceinline__ void dft(double2* v) {
-    if constexpr (R == 16) dft16<INV>(v);
+    if constexpr (R == 25) dft_ct<5, 5, INV>(v);
+    else if constexpr (R == 24) dft_ct<8, 3, INV>(v);
+    else if constexpr (R == 20) dft_ct<4, 5, INV>(v);
+    else if constexpr (R == 18) dft_ct<2, 9, INV>(v);
+    else if constexpr (R == 16) dft16<INV>(v);
     else if constexpr (R == 15) dft_ct<3, 5, INV>(v);
     else if constexpr (R == 14) dft_ct<2, 7, INV>(v);
     else if constexpr (R == 12) dft12<INV>(v);
@@ -956,6 +1028,16 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 112: return axis_launch<112, 14, 8>(a, st);
         case 120: return axis_launch<120, 15, 8>(a, st);
         case 128: return axis_launch<128, 16, 8>(a, st);
+        // lengths the slab pass does not serve (a plane would not fit in LDS, or no plane
+        // factorisation): one line per 32-line tile is enough here
+        case 20: return axis_launch<20, 5, 4>(a, st);
+        case 36: return axis_launch<36, 9, 4>(a, st);
+        case 60: return axis_launch<60, 15, 4>(a, st);
+        case 144: return axis_launch<144, 18, 8>(a, st);
+        case 160: return axis_launch<160, 20, 8>(a, st);
+        case 192: return axis_launch<192, 24, 8>(a, st);
+        case 200: return axis_launch<200, 25, 8>(a, st);
+        case 256: return axis_launch<256, 16, 16>(a, st);
         default: return hipErrorNotSupported;
     }
 }

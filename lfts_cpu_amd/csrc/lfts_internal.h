@@ -91,14 +91,15 @@ struct LineArgs {
     double* outR;
     const double* mul;
     const int* skip;      // device flag: when set, the launch returns at once (or null)
-    // LINE_CONV on a line length the slab pass serves: fast = 1 runs the register-FFT
+    // LINE_CONV on a line length the axis kernel serves: fast = 1 runs the register-FFT
     // axis kernel (natural-order kernel values gnat, g(k)/n) instead of the generic one
     const double* gnat;
     int fast;
 };
 hipError_t launch_line(const LineArgs& a, hipStream_t st);
-// the register-FFT convolution along one axis (lfts_fused.hip); hipErrorNotSupported
-// when the line length is not one of SLAB_EDGES
+// the register-FFT convolution along one axis (lfts_fused.hip): line lengths 16, 20, 24,
+// 32, 36, 40, 48, 56, 60, 64, 72, 80, 96, 112, 120, 128, 144, 160, 192, 200 and 256;
+// hipErrorNotSupported for any other
 hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st);
 
 // ---- fused slab pass over replicas with a cubic mesh of edge n -----------------------

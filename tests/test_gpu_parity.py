@@ -257,7 +257,13 @@ def test_fused_slab_path_against_oracle_and_generic_path(gpu, monkeypatch, n, N,
 
 
 @pytest.mark.parametrize("m,Lb,N,NA", [((32, 16, 24), (3.5, 2.8, 3.1), 10, 4), ((128, 64, 56), (5.0, 3.9, 3.3), 6, 3),
-                                       ((16, 12, 120), (2.5, 2.2, 6.0), 7, 7)])
+                                       ((16, 12, 120), (2.5, 2.2, 6.0), 7, 7),
+                                       ((20, 36, 60), (2.6, 3.4, 4.4), 8, 3),     # lengths only the axis kernel serves
+                                       ((144, 10, 8), (6.5, 2.4, 2.2), 6, 2),
+                                       ((8, 160, 6), (2.2, 6.9, 2.0), 7, 4),
+                                       ((6, 9, 192), (2.0, 2.5, 7.4), 6, 0),
+                                       ((200, 6, 8), (7.6, 2.0, 2.2), 5, 3),
+                                       ((8, 6, 256), (2.2, 2.0, 8.4), 6, 3)])
 def test_axis_kernel_against_generic_line_kernel(gpu, monkeypatch, m, Lb, N, NA):
     """Line-path meshes whose axes have slab-edge lengths run the register-FFT axis kernel;
     LFTS_PATH=generic runs the generic line kernel on the same input. Pairs, lnQ and phi
