@@ -1038,6 +1038,9 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 192: return axis_launch<192, 24, 8>(a, st);
         case 200: return axis_launch<200, 25, 8>(a, st);
         case 256: return axis_launch<256, 16, 16>(a, st);
+        case 30: return axis_launch<30, 10, 3>(a, st);
+        case 50: return axis_launch<50, 10, 5>(a, st);
+        case 100: return axis_launch<100, 25, 4>(a, st);
         default: return hipErrorNotSupported;
     }
 }
