@@ -924,16 +924,16 @@ hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, 
 
 // ---- convolution along one axis with register FFTs (line path) ---------------------------
 // For meshes the slab pass does not serve, a contour step is three passes, one per axis
-// (src/step.cc:35-65 as Cz, Cy, Cx; the last multiplies by h). A pass whose line length is
-// one of the slab edges runs here: 32 complex lines per workgroup, each line the two real
+// (src/step.cc:35-65 as Cz, Cy, Cx; the last multiplies by h). A pass whose line length
+// launch_axis_conv serves runs here: 16 complex lines per workgroup, each line the two real
 // fields of a pair at one position (field A + i field B), held by R2 threads with R1
 // elements each and convolved by conv_line with the tables in LDS. For x- and y-lines the
 // lanes of a wave take consecutive lines (adjacent in memory), so each load or store
-// covers a 256-byte run; for z-lines (contiguous) the R2 threads of a line are adjacent.
+// covers a 128-byte run; for z-lines (contiguous) the R2 threads of a line are adjacent.
 // The finish follows the generic kernel's rules (plain, times h, tangent, adjoint).
 namespace {
 
-constexpr int AX_LINES = 32;
+constexpr int AX_LINES = 16;  // 16 lines per workgroup: 128-byte runs, more workgroups per CU
 
 __device__ __forceinline__ long axis_line_base(const LineArgs& a, long line) {
     if (a.axis == 2) return line * a.m2;
