@@ -1021,13 +1021,13 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 40: return axis_launch<40, 5, 8>(a, st);
         case 48: return axis_launch<48, 12, 4>(a, st);
         case 56: return axis_launch<56, 7, 8>(a, st);
-        case 64: return axis_launch<64, 8, 8>(a, st);
+        case 64: return axis_launch<64, 4, 16>(a, st);  // 16 threads per line: more waves
         case 72: return axis_launch<72, 9, 8>(a, st);
         case 80: return axis_launch<80, 10, 8>(a, st);
         case 96: return axis_launch<96, 12, 8>(a, st);
         case 112: return axis_launch<112, 14, 8>(a, st);
         case 120: return axis_launch<120, 15, 8>(a, st);
-        case 128: return axis_launch<128, 16, 8>(a, st);
+        case 128: return axis_launch<128, 8, 16>(a, st);
         // lengths the slab pass does not serve (a plane would not fit in LDS, or no plane
         // factorisation): one line per 32-line tile is enough here
         case 20: return axis_launch<20, 5, 4>(a, st);
