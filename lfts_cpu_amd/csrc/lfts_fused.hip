@@ -1002,11 +1002,14 @@ __global__ __launch_bounds__(AX_LINES * R2) void k_axis(const LineArgs a) {
 
 namespace {
 
-template <int E, int R1, int R2>
+// (R1, R2) for z-lines; (R1X, R2X) for x- and y-lines, where a line's threads sit in
+// different waves, so stage-2 DFTs over fewer threads than R2X idle whole waves instead of
+// lanes and more threads per line can pay (128 = 8 x 16 there)
+template <int E, int R1, int R2, int R1X = R1, int R2X = R2>
 hipError_t axis_launch(const LineArgs& a, hipStream_t st) {
     const unsigned nblk = (unsigned)((a.nlines + AX_LINES - 1) / AX_LINES);
     if (a.axis == 2) hipLaunchKernelGGL((k_axis<E, R1, R2, true>), dim3(nblk), dim3(AX_LINES * R2), 0, st, a);
-    else hipLaunchKernelGGL((k_axis<E, R1, R2, false>), dim3(nblk), dim3(AX_LINES * R2), 0, st, a);
+    else hipLaunchKernelGGL((k_axis<E, R1X, R2X, false>), dim3(nblk), dim3(AX_LINES * R2X), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1021,13 +1024,13 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 40: return axis_launch<40, 5, 8>(a, st);
         case 48: return axis_launch<48, 12, 4>(a, st);
         case 56: return axis_launch<56, 7, 8>(a, st);
-        case 64: return axis_launch<64, 4, 16>(a, st);  // 16 threads per line: more waves
+        case 64: return axis_launch<64, 8, 8>(a, st);
         case 72: return axis_launch<72, 9, 8>(a, st);
         case 80: return axis_launch<80, 10, 8>(a, st);
         case 96: return axis_launch<96, 12, 8>(a, st);
         case 112: return axis_launch<112, 14, 8>(a, st);
         case 120: return axis_launch<120, 15, 8>(a, st);
-        case 128: return axis_launch<128, 8, 16>(a, st);
+        case 128: return axis_launch<128, 16, 8, 8, 16>(a, st);
         // lengths the slab pass does not serve (a plane would not fit in LDS, or no plane
         // factorisation): one line per 32-line tile is enough here
         case 20: return axis_launch<20, 5, 4>(a, st);
