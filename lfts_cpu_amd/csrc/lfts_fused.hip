@@ -412,6 +412,14 @@ __device__ __forceinline__ double2 root_cs(int e) {
             -0.951056516295153572116, -0.844327925502015078549, -0.684547105928688673732, -0.481753674101715274987,
             -0.248689887164854788242};
         return make_double2(c[e], s[e]);
+    } else if constexpr (R == 6) {
+        constexpr double c[6] = {
+            1.0, 0.5, -0.5, -1.0,
+            -0.5, 0.5};
+        constexpr double s[6] = {
+            0.0, 0.866025403784438646764, 0.866025403784438646764, 0.0,
+            -0.866025403784438646764, -0.866025403784438646764};
+        return make_double2(c[e], s[e]);
     } else {
         static_assert(R == 9, "no twiddle table for this length");
         return make_double2(1.0, 0.0);
@@ -495,6 +503,7 @@ __device__ __forceinline__ void dft(double2* v) {
     else if constexpr (R == 9) dft_ct<3, 3, INV>(v);
     else if constexpr (R == 8) dft8<INV>(v);
     else if constexpr (R == 7) dft7<INV>(v);
+    else if constexpr (R == 6) dft_ct<2, 3, INV>(v);
     else if constexpr (R == 5) dft5<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
     else if constexpr (R == 3) dft3<INV>(v[0], v[1], v[2]);
@@ -1022,13 +1031,13 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 24: return axis_launch<24, 3, 8>(a, st);
         case 32: return axis_launch<32, 8, 4>(a, st);
         case 40: return axis_launch<40, 5, 8>(a, st);
-        case 48: return axis_launch<48, 12, 4>(a, st);
+        case 48: return axis_launch<48, 12, 4, 3, 16>(a, st);
         case 56: return axis_launch<56, 7, 8>(a, st);
         case 64: return axis_launch<64, 8, 8>(a, st);
         case 72: return axis_launch<72, 9, 8>(a, st);
-        case 80: return axis_launch<80, 10, 8>(a, st);
-        case 96: return axis_launch<96, 12, 8>(a, st);
-        case 112: return axis_launch<112, 14, 8>(a, st);
+        case 80: return axis_launch<80, 10, 8, 5, 16>(a, st);
+        case 96: return axis_launch<96, 12, 8, 6, 16>(a, st);
+        case 112: return axis_launch<112, 14, 8, 7, 16>(a, st);
         case 120: return axis_launch<120, 15, 8>(a, st);
         case 128: return axis_launch<128, 16, 8, 8, 16>(a, st);
         // lengths the slab pass does not serve (a plane would not fit in LDS, or no plane
@@ -1036,9 +1045,9 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 20: return axis_launch<20, 5, 4>(a, st);
         case 36: return axis_launch<36, 9, 4>(a, st);
         case 60: return axis_launch<60, 15, 4>(a, st);
-        case 144: return axis_launch<144, 18, 8>(a, st);
-        case 160: return axis_launch<160, 20, 8>(a, st);
-        case 192: return axis_launch<192, 24, 8>(a, st);
+        case 144: return axis_launch<144, 18, 8, 9, 16>(a, st);
+        case 160: return axis_launch<160, 20, 8, 10, 16>(a, st);
+        case 192: return axis_launch<192, 24, 8, 12, 16>(a, st);
         case 200: return axis_launch<200, 25, 8>(a, st);
         case 256: return axis_launch<256, 16, 16>(a, st);
         case 30: return axis_launch<30, 10, 3>(a, st);

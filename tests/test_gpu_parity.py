@@ -265,7 +265,11 @@ def test_fused_slab_path_against_oracle_and_generic_path(gpu, monkeypatch, n, N,
                                        ((200, 6, 8), (7.6, 2.0, 2.2), 5, 3),
                                        ((8, 6, 256), (2.2, 2.0, 8.4), 6, 3),
                                        ((30, 50, 12), (3.0, 4.2, 2.4), 7, 2),
-                                       ((100, 8, 6), (5.5, 2.2, 2.0), 6, 4)])
+                                       ((100, 8, 6), (5.5, 2.2, 2.0), 6, 4),
+                                       ((48, 80, 12), (3.7, 5.0, 2.4), 6, 2),    # x/y lines of 16 threads
+                                       ((96, 112, 8), (5.2, 5.6, 2.2), 5, 3),
+                                       ((144, 160, 6), (6.5, 6.9, 2.0), 5, 2),
+                                       ((192, 128, 4), (7.4, 6.0, 1.8), 4, 2)])
 def test_axis_kernel_against_generic_line_kernel(gpu, monkeypatch, m, Lb, N, NA):
     """Line-path meshes whose axes have slab-edge lengths run the register-FFT axis kernel;
     LFTS_PATH=generic runs the generic line kernel on the same input. Pairs, lnQ and phi
