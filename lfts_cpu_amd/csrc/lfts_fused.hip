@@ -1053,6 +1053,29 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 30: return axis_launch<30, 10, 3>(a, st);
         case 50: return axis_launch<50, 10, 5>(a, st);
         case 100: return axis_launch<100, 25, 4>(a, st);
+        // further lengths with factors 2, 3, 5 and 7 (R1 x R2 from the register DFTs)
+        case 18: return axis_launch<18, 3, 6>(a, st);
+        case 28: return axis_launch<28, 7, 4>(a, st);
+        case 42: return axis_launch<42, 7, 6>(a, st);
+        case 54: return axis_launch<54, 9, 6>(a, st);
+        case 70: return axis_launch<70, 7, 10>(a, st);
+        case 84: return axis_launch<84, 12, 7>(a, st);
+        case 90: return axis_launch<90, 9, 10>(a, st);
+        case 98: return axis_launch<98, 14, 7>(a, st);
+        case 108: return axis_launch<108, 12, 9>(a, st);
+        case 126: return axis_launch<126, 14, 9>(a, st);
+        case 140: return axis_launch<140, 14, 10>(a, st);
+        case 150: return axis_launch<150, 15, 10>(a, st);
+        case 162: return axis_launch<162, 18, 9>(a, st);
+        case 168: return axis_launch<168, 14, 12>(a, st);
+        case 180: return axis_launch<180, 15, 12>(a, st);
+        case 196: return axis_launch<196, 14, 14>(a, st);
+        case 210: return axis_launch<210, 15, 14>(a, st);
+        case 216: return axis_launch<216, 18, 12>(a, st);
+        case 224: return axis_launch<224, 14, 16>(a, st);
+        case 240: return axis_launch<240, 15, 16>(a, st);
+        case 250: return axis_launch<250, 25, 10>(a, st);
+        case 252: return axis_launch<252, 18, 14>(a, st);
         default: return hipErrorNotSupported;
     }
 }

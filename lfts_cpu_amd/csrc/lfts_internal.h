@@ -97,9 +97,9 @@ struct LineArgs {
     int fast;
 };
 hipError_t launch_line(const LineArgs& a, hipStream_t st);
-// the register-FFT convolution along one axis (lfts_fused.hip): line lengths 16, 20, 24,
-// 30, 32, 36, 40, 48, 50, 56, 60, 64, 72, 80, 96, 100, 112, 120, 128, 144, 160, 192, 200
-// and 256; hipErrorNotSupported for any other
+// the register-FFT convolution along one axis (lfts_fused.hip): every even line length
+// from 16 to 256 whose prime factors are among 2, 3, 5 and 7; hipErrorNotSupported for
+// any other length
 hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st);
 
 // ---- fused slab pass over replicas with a cubic mesh of edge n -----------------------

@@ -269,7 +269,23 @@ def test_fused_slab_path_against_oracle_and_generic_path(gpu, monkeypatch, n, N,
                                        ((48, 80, 12), (3.7, 5.0, 2.4), 6, 2),    # x/y lines of 16 threads
                                        ((96, 112, 8), (5.2, 5.6, 2.2), 5, 3),
                                        ((144, 160, 6), (6.5, 6.9, 2.0), 5, 2),
-                                       ((192, 128, 4), (7.4, 6.0, 1.8), 4, 2)])
+                                       ((192, 128, 4), (7.4, 6.0, 1.8), 4, 2),
+                                       # the remaining even lengths with factors 2, 3, 5, 7 (x/y and z lines)
+                                       ((18, 28, 42), (2.3, 2.8, 3.6), 6, 3),
+                                       ((54, 70, 12), (3.8, 4.4, 2.4), 5, 2),
+                                       ((84, 90, 6), (4.8, 5.0, 2.0), 5, 3),
+                                       ((98, 108, 4), (5.2, 5.5, 1.8), 4, 2),
+                                       ((126, 140, 4), (5.9, 6.2, 1.8), 4, 1),
+                                       ((150, 162, 4), (6.4, 6.7, 1.8), 4, 3),
+                                       ((168, 180, 2), (6.8, 7.0, 1.5), 4, 2),
+                                       ((196, 210, 2), (7.3, 7.6, 1.5), 4, 2),
+                                       ((216, 224, 2), (7.6, 7.8, 1.5), 4, 2),
+                                       ((240, 250, 2), (8.0, 8.2, 1.5), 4, 2),
+                                       ((8, 6, 210), (2.2, 2.0, 7.6), 5, 2),
+                                       ((6, 8, 168), (2.0, 2.2, 6.8), 5, 3),
+                                       ((4, 6, 250), (1.8, 2.0, 8.2), 5, 2),
+                                       ((4, 8, 216), (1.8, 2.2, 7.6), 5, 4),
+                                       ((252, 6, 4), (8.3, 2.0, 1.8), 4, 2)])
 def test_axis_kernel_against_generic_line_kernel(gpu, monkeypatch, m, Lb, N, NA):
     """Line-path meshes whose axes have slab-edge lengths run the register-FFT axis kernel;
     LFTS_PATH=generic runs the generic line kernel on the same input. Pairs, lnQ and phi
