@@ -1041,7 +1041,7 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 120: return axis_launch<120, 15, 8>(a, st);
         case 128: return axis_launch<128, 16, 8, 8, 16>(a, st);
         // lengths the slab pass does not serve (a plane would not fit in LDS, or no plane
-        // factorisation): one line per 32-line tile is enough here
+        // factorisation): the axis kernel keeps only a 16-line tile in LDS
         case 20: return axis_launch<20, 5, 4>(a, st);
         case 36: return axis_launch<36, 9, 4>(a, st);
         case 60: return axis_launch<60, 15, 4>(a, st);
