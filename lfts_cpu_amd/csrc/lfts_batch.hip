@@ -59,6 +59,8 @@ __device__ __forceinline__ double* part_row(double* part, int slot) {
     return part + ((long)slot * MAX_NB + blockIdx.x) * PSTRIDE;
 }
 
+constexpr int PHI_UNROLL = 8;  // beads per group in k_phi's pass over the stored pairs
+
 #define FOR_POINTS(r, R) for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < (R).M; r += (long)(R).nb * blockDim.x)
 #define REPLICA(R)                              \
     const Rep R = reps[blockIdx.y];             \
@@ -88,22 +90,39 @@ __global__ __launch_bounds__(256) void k_q_sum(const Rep* __restrict__ reps, dou
 }
 
 // scal[rid][off + c] = sum over the replica's partials (divided by M when asked), all
-// ncomp (<= PSTRIDE) components together
+// ncomp (<= PSTRIDE) components together. Thread t adds rows t, t + 256, ... (< nb) in
+// that order. All of its rows are loaded before the first add: the partials were written
+// by workgroups on every XCD, so each load is a trip to memory, and a load-add loop paid
+// one trip per row. Row indexes from nb up are clamped to nb - 1 (in bounds, never added),
+// and every row is PSTRIDE doubles wide, so the loads carry no condition; components from
+// ncomp up are summed but never read.
 __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, const double* __restrict__ part,
                                                 int ncomp, int div_by_M, double* __restrict__ scal, int off) {
+    constexpr int ROWS = MAX_NB / 256;  // rows per thread (launched with 256 threads)
+    static_assert(MAX_NB % 256 == 0, "whole rows per thread");
     __shared__ double red[4 * PSTRIDE];
     __shared__ double res[PSTRIDE];
     const int slot = blockIdx.x;
     const Rep R = reps[slot];
     if (R.skip && *R.skip) return;
     const double* p = part + (long)slot * MAX_NB * PSTRIDE;
+    const int nb = R.nb;  // 1 <= nb <= MAX_NB
+    double v[ROWS][PSTRIDE];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+        const int b = (int)threadIdx.x + 256 * k;
+        const long row = (b < nb) ? b : nb - 1;
+#pragma unroll
+        for (int c = 0; c < PSTRIDE; c++) v[k][c] = p[row * PSTRIDE + c];
+    }
     double s[PSTRIDE];
 #pragma unroll
     for (int c = 0; c < PSTRIDE; c++) s[c] = 0.0;
-    for (int b = threadIdx.x; b < R.nb; b += blockDim.x)
 #pragma unroll
-        for (int c = 0; c < PSTRIDE; c++)
-            if (c < ncomp) s[c] += p[(long)b * PSTRIDE + c];
+    for (int k = 0; k < ROWS; k++)
+        if ((int)threadIdx.x + 256 * k < nb)
+#pragma unroll
+            for (int c = 0; c < PSTRIDE; c++) s[c] += v[k][c];
     block_sums<PSTRIDE>(s, ncomp, red, res);
     if ((int)threadIdx.x < ncomp) {
         const double v = res[threadIdx.x];
@@ -128,23 +147,45 @@ __global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const
     phi_sums_used(N, NA, u);
     FOR_POINTS(r, R) {
         double s[4] = {0.0, 0.0, 0.0, 0.0};
-        if (fused) {
-            for (int a = 0; a < 4; a++)
-                if (u[a]) s[a] = R.phacc[(long)a * M + r];
+        if (fused) {  // phacc holds 4 M doubles: load all four sums, keep the used ones
+            double v[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) v[a] = R.phacc[(long)a * M + r];
+#pragma unroll
+            for (int a = 0; a < 4; a++) s[a] = u[a] ? v[a] : 0.0;
         } else {
+            // Beads in groups of PHI_UNROLL, each group's loads issued before its products:
+            // one load pair per bead in flight did not cover the memory latency.
+            constexpr int U = PHI_UNROLL;
             bool t[4] = {false, false, false, false};
-            for (int j = i0 + 1; j <= N; j++) {
-                const int a = (j <= NA) ? 0 : 1;
-                const double x = rep_pair(R, j)[r], y = rep_pair(R, N + 1 - j)[M + r];  // q_j, q+_j
+            auto add = [&](int a, double x, double y) {
                 s[a] = t[a] ? fma(x, y, s[a]) : x * y;
                 t[a] = true;
+            };
+            int j = i0 + 1;
+            for (; j + U - 1 <= N; j += U) {
+                double x[U], y[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    x[u] = rep_pair(R, j + u)[r];              // q_j
+                    y[u] = rep_pair(R, N + 1 - j - u)[M + r];  // q+_j
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) add((j + u <= NA) ? 0 : 1, x[u], y[u]);
             }
-            for (int j = N - i0; j >= 1; j--) {
-                const int a = (j <= NA) ? 2 : 3;
-                const double x = rep_pair(R, N + 1 - j)[M + r], y = rep_pair(R, j)[r];  // q+_j, q_j
-                s[a] = t[a] ? fma(x, y, s[a]) : x * y;
-                t[a] = true;
+            for (; j <= N; j++) add((j <= NA) ? 0 : 1, rep_pair(R, j)[r], rep_pair(R, N + 1 - j)[M + r]);
+            j = N - i0;
+            for (; j - U + 1 >= 1; j -= U) {
+                double x[U], y[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    x[u] = rep_pair(R, N + 1 - j + u)[M + r];  // q+_j
+                    y[u] = rep_pair(R, j - u)[r];              // q_j
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) add((j - u <= NA) ? 2 : 3, x[u], y[u]);
             }
+            for (; j >= 1; j--) add((j <= NA) ? 2 : 3, rep_pair(R, N + 1 - j)[M + r], rep_pair(R, j)[r]);
         }
         double pA = s[0] + s[2], pB = s[1] + s[3];
         if (N & 1) {  // middle bead: both fields of pair i0
@@ -159,43 +200,88 @@ __global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const
     }
 }
 
-// src/anderson.cc:35-55: DD[0][m] = <Dh0, Dh_m>, Dh0 = phi+ - 1 (m = 0 uses Dh0 itself)
+// src/anderson.cc:35-55: DD[0][m] = <Dh0, Dh_m>, Dh0 = phi+ - 1 (m = 0 uses Dh0 itself).
+// One instantiation per history depth NH: a point's NH history loads are unconditional and
+// all in flight before the first product (under a run-time `m <= nh` test each load was
+// followed by its own wait).
+template <int NH>
+__device__ __forceinline__ void am_dots_row(const Rep& R, double* redn, double* out) {
+    const long M = R.M;
+    double acc[NH + 1];
+#pragma unroll
+    for (int m = 0; m <= NH; m++) acc[m] = 0.0;
+    FOR_POINTS(r, R) {
+        const double d0 = R.w[3 * M + r] - 1.0;
+        double v[NH + 1];
+#pragma unroll
+        for (int m = 1; m <= NH; m++) v[m] = R.Dh[(long)m * M + r];
+        acc[0] += d0 * d0;
+#pragma unroll
+        for (int m = 1; m <= NH; m++) acc[m] += d0 * v[m];
+    }
+    block_sums<NH + 1>(acc, NH + 1, redn, out);
+}
+
 __global__ __launch_bounds__(256) void k_am_dots(const Rep* __restrict__ reps, const Coef* __restrict__ coef,
                                                  double* __restrict__ part) {
     __shared__ double redn[4 * (MAX_HIST + 1)];
     REPLICA(R);
-    const int nh = coef[R.id].nh_dots;
-    const long M = R.M;
-    double acc[MAX_HIST + 1];
-#pragma unroll
-    for (int m = 0; m <= MAX_HIST; m++) acc[m] = 0.0;
-    FOR_POINTS(r, R) {
-        const double d0 = R.w[3 * M + r] - 1.0;
-        acc[0] += d0 * d0;
-#pragma unroll
-        for (int m = 1; m <= MAX_HIST; m++)
-            if (m <= nh) acc[m] += d0 * R.Dh[(long)m * M + r];
+    double* out = part_row(part, blockIdx.y);
+    static_assert(MAX_HIST == 15, "one case per history depth");
+    switch (coef[R.id].nh_dots) {  // 0 <= nh_dots <= nhMax <= MAX_HIST (checked at creation)
+#define AM_DOTS_CASE(k) \
+    case k: am_dots_row<k>(R, redn, out); break;
+        AM_DOTS_CASE(0) AM_DOTS_CASE(1) AM_DOTS_CASE(2) AM_DOTS_CASE(3) AM_DOTS_CASE(4) AM_DOTS_CASE(5)
+        AM_DOTS_CASE(6) AM_DOTS_CASE(7) AM_DOTS_CASE(8) AM_DOTS_CASE(9) AM_DOTS_CASE(10) AM_DOTS_CASE(11)
+        AM_DOTS_CASE(12) AM_DOTS_CASE(13) AM_DOTS_CASE(14) AM_DOTS_CASE(15)
+#undef AM_DOTS_CASE
+        default: break;
     }
-    block_sums<MAX_HIST + 1>(acc, nh + 1, redn, part_row(part, blockIdx.y));
 }
 
 // src/anderson.cc:57-89: w+ += lambda Dh0 + sum_n C_n((wh_n + lambda Dh_n) - (wh0 + lambda Dh0)),
 // then (Dh0, wh0) become history slot `slot`.
-__global__ __launch_bounds__(256) void k_am_update(const Rep* __restrict__ reps, const Coef* __restrict__ coef) {
-    REPLICA(R);
-    const Coef& c = coef[R.id];
+// One instantiation per history depth NH, so a point's 2 NH history loads are all issued
+// before the first update term; the terms are added in the order n = 1 .. NH.
+template <int NH>
+__device__ __forceinline__ void am_update_row(const Rep& R, const Coef& c) {
     const double lambda = c.lambda;
-    const int nh = c.nh, slot = c.slot;
+    const int slot = c.slot;
     const long M = R.M;
+    double C[NH + 1];
+#pragma unroll
+    for (int n = 1; n <= NH; n++) C[n] = c.C[n - 1];
     FOR_POINTS(r, R) {
         const double d0 = R.w[3 * M + r] - 1.0;
         const double w0 = R.w[M + r];
+        double vw[NH + 1], vd[NH + 1];
+#pragma unroll
+        for (int n = 1; n <= NH; n++) {
+            vw[n] = R.wh[(long)n * M + r];
+            vd[n] = R.Dh[(long)n * M + r];
+        }
         const double base = w0 + lambda * d0;
         double wn = base;
-        for (int n = 1; n <= nh; n++) wn += c.C[n - 1] * ((R.wh[(long)n * M + r] + lambda * R.Dh[(long)n * M + r]) - base);
+#pragma unroll
+        for (int n = 1; n <= NH; n++) wn += C[n] * ((vw[n] + lambda * vd[n]) - base);
         R.w[M + r] = wn;
         R.Dh[(long)slot * M + r] = d0;
         R.wh[(long)slot * M + r] = w0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_am_update(const Rep* __restrict__ reps, const Coef* __restrict__ coef) {
+    REPLICA(R);
+    const Coef& c = coef[R.id];
+    switch (c.nh) {  // 0 <= nh <= nhMax <= MAX_HIST (checked at creation)
+#define AM_UPDATE_CASE(k) \
+    case k: am_update_row<k>(R, c); break;
+        AM_UPDATE_CASE(0) AM_UPDATE_CASE(1) AM_UPDATE_CASE(2) AM_UPDATE_CASE(3) AM_UPDATE_CASE(4)
+        AM_UPDATE_CASE(5) AM_UPDATE_CASE(6) AM_UPDATE_CASE(7) AM_UPDATE_CASE(8) AM_UPDATE_CASE(9)
+        AM_UPDATE_CASE(10) AM_UPDATE_CASE(11) AM_UPDATE_CASE(12) AM_UPDATE_CASE(13) AM_UPDATE_CASE(14)
+        AM_UPDATE_CASE(15)
+#undef AM_UPDATE_CASE
+        default: break;
     }
 }
 
