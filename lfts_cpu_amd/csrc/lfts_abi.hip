@@ -352,10 +352,14 @@ int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep*
 
 // One propagator evaluation (src/diblock.cc:30-60) of every active replica; Q of
 // replica id lands in d_scal[id][0].
-int calc_concs_async(lfts_batch* b, bool store_all) {
+int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, bool dots) {
     const int nact = (int)b->act.size();
     if (nact == 0) return LFTS_OK;
-    if (b->ctx.size() == 1 && b->ctx[0]->ck_stride) return ckpt_calc_concs(b->ctx[0]);
+    if (b->ctx.size() == 1 && b->ctx[0]->ck_stride) {  // checkpointed storage: its own phi pass
+        if (int rc = ckpt_calc_concs(b->ctx[0])) return rc;
+        if (dots) BHIPCHK(b, lfts::launch_am_dots(b->d_ract, nact, b->nb_max, b->d_coef, b->d_part, b->st));
+        return LFTS_OK;
+    }
     // One replica alone runs latency-bound passes: it fuses the density sums into the
     // second half of its chain (no separate pass over the stored pairs). A set moves the
     // same bytes either way and keeps the leaner kernel.
@@ -368,7 +372,7 @@ int calc_concs_async(lfts_batch* b, bool store_all) {
             if (int rc = get_event(b, e)) return rc;
         BHIPCHK(b, hipEventRecord(ev.pe0, b->st));
     }
-    BHIPCHK(b, lfts::launch_prepare_h(b->d_ract, nact, b->nb_max, b->st));
+    if (!h_ready) BHIPCHK(b, lfts::launch_prepare_h(b->d_ract, nact, b->nb_max, b->st));
     if (rec) BHIPCHK(b, hipEventRecord(ev.ch0, b->st));
     long long npasses = 0;
     // slab passes: i = 0 prologue, i = 1..N-1 contour steps (src/diblock.cc:43). With
@@ -414,7 +418,8 @@ int calc_concs_async(lfts_batch* b, bool store_all) {
     if (rec) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
     BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_part, b->st));
     BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, 1, 1, b->d_scal, 0, b->st));
-    BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, fuse, b->st));
+    BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, fuse, dots ? b->d_coef : nullptr, b->d_part,
+                                b->st));
     if (rec) {
         BHIPCHK(b, hipEventRecord(ev.pe1, b->st));
         ev.npasses = npasses;
@@ -468,19 +473,22 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
         if (int rc = upload_coef(b)) return rc;
     }
     std::vector<int> cur;
+    // From the second iteration on, every member still active was in the previous
+    // update, which also wrote its Boltzmann factors (with_h).
+    bool h_ready = false;
     while (!active.empty()) {
         if (active != cur) {
             if (int rc = set_active(b, active)) return rc;
             cur = active;
         }
-        if (int rc = calc_concs_async(b)) return rc;
+        // the evaluation's phi pass also writes the Gram row partials (dots)
+        if (int rc = calc_concs_async(b, false, h_ready, true)) return rc;
         int nh_max = 0;
         for (int id : active) {
             const lfts_ctx* c = b->ctx[id];
             nh_max = std::max(nh_max, (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax);
         }
         const int nact = (int)b->act.size();
-        BHIPCHK(b, lfts::launch_am_dots(b->d_ract, nact, b->nb_max, b->d_coef, b->d_part, b->st));
         BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, nh_max + 1, 0, b->d_scal, 0, b->st));
         if (int rc = read_scal(b)) return rc;
         for (int id : active) {
@@ -511,7 +519,8 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             co.slot = 1 + (k[id] - 1) % c->nhMax;
         }
         if (int rc = upload_coef(b)) return rc;
-        BHIPCHK(b, lfts::launch_am_update(b->d_ract, nact, b->nb_max, b->d_coef, b->st));
+        BHIPCHK(b, lfts::launch_am_update(b->d_ract, nact, b->nb_max, b->d_coef, 1, b->st));
+        h_ready = true;
         std::vector<int> still;
         for (int id : active) {
             lfts_ctx* c = b->ctx[id];
@@ -584,7 +593,7 @@ int am_iteration(lfts_ctx* c) {
     BHIPCHK(b, lfts::launch_am_dots(b->d_ract, 1, b->nb_max, b->d_coef, b->d_part, b->st));
     BHIPCHK(b, lfts::launch_reduce(b->d_ract, 1, b->d_part, c->nhMax + 1, 0, b->d_scal, 0, b->st));
     BHIPCHK(b, lfts::launch_am_solve(c->am, b->d_scal, b->d_coef, c->pow09, c->M, c->nhMax, b->st));
-    BHIPCHK(b, lfts::launch_am_update(b->d_ract, 1, b->nb_max, b->d_coef, b->st));
+    BHIPCHK(b, lfts::launch_am_update(b->d_ract, 1, b->nb_max, b->d_coef, 0, b->st));
     BHIPCHK(b, lfts::launch_am_advance(c->am, b->st));
     return LFTS_OK;
 }

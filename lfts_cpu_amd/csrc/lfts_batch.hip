@@ -136,21 +136,38 @@ __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, co
 // i0+1..N ascending (block A/B), s[2]/s[3] over beads N-i0..1 descending, each started by
 // the first product and continued with fma; the middle bead of an odd N comes last. The
 // stored-pair evaluation repeats that order, so phi is bit-identical either way.
-__global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const double* __restrict__ scal,
-                                             int fuse) {
-    REPLICA(R);
+//
+// NH >= 0: the same pass also forms the Anderson Gram row partials <Dh0, Dh_m>, m = 0..NH
+// (Dh0 = phi+ - 1, src/anderson.cc:35-55) over the same points in the same order as
+// k_am_dots, so the partial rows are identical and the separate pass is saved.
+template <int NH>
+__device__ __forceinline__ void phi_row(const Rep& R, const double* scal, int fuse, double* redn, double* out) {
     const double Q = scal[(long)R.id * SSTRIDE];
     const long M = R.M;
     const int N = R.N, NA = R.NA, i0 = (N + 1) / 2;
     const bool fused = fused_phi(R, fuse);
     bool u[4];
     phi_sums_used(N, NA, u);
+    constexpr int NG = NH < 0 ? 1 : NH + 1;
+    double acc[NG];
+#pragma unroll
+    for (int m = 0; m < NG; m++) acc[m] = 0.0;
     FOR_POINTS(r, R) {
+        // history slots of the Gram row: loaded with the sums (finish) or after the bead
+        // loops (stored pairs), so they are never live beside a group of bead loads
+        double hist[NG];
+        auto load_hist = [&]() {
+            if constexpr (NH > 0) {
+#pragma unroll
+                for (int m = 1; m <= NH; m++) hist[m] = R.Dh[(long)m * M + r];
+            }
+        };
         double s[4] = {0.0, 0.0, 0.0, 0.0};
         if (fused) {  // phacc holds 4 M doubles: load all four sums, keep the used ones
             double v[4];
 #pragma unroll
             for (int a = 0; a < 4; a++) v[a] = R.phacc[(long)a * M + r];
+            load_hist();
 #pragma unroll
             for (int a = 0; a < 4; a++) s[a] = u[a] ? v[a] : 0.0;
         } else {
@@ -186,6 +203,7 @@ __global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const
                 for (int u = 0; u < U; u++) add((j - u <= NA) ? 2 : 3, x[u], y[u]);
             }
             for (; j >= 1; j--) add((j <= NA) ? 2 : 3, rep_pair(R, N + 1 - j)[M + r], rep_pair(R, j)[r]);
+            load_hist();
         }
         double pA = s[0] + s[2], pB = s[1] + s[3];
         if (N & 1) {  // middle bead: both fields of pair i0
@@ -195,10 +213,41 @@ __global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const
         }
         const double phA = pA / (R.N * Q * R.h[r]);
         const double phB = pB / (R.N * Q * R.h[r + M]);
+        const double pp = phA + phB;
         R.w[2 * M + r] = phA - phB;
-        R.w[3 * M + r] = phA + phB;
+        R.w[3 * M + r] = pp;
+        if constexpr (NH >= 0) {
+            const double d0 = pp - 1.0;
+            acc[0] += d0 * d0;
+#pragma unroll
+            for (int m = 1; m <= NH; m++) acc[m] += d0 * hist[m];
+        }
+    }
+    if constexpr (NH >= 0) block_sums<NH + 1>(acc, NH + 1, redn, out);
+}
+
+// With coef, also the Gram row partials of each member's next Anderson iteration
+// (history depth coef[id].nh_dots) into part, as k_am_dots writes them.
+__global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const double* __restrict__ scal,
+                                             int fuse, const Coef* __restrict__ coef, double* __restrict__ part) {
+    __shared__ double redn[4 * (MAX_HIST + 1)];
+    REPLICA(R);
+    if (!coef) {
+        phi_row<-1>(R, scal, fuse, redn, nullptr);
+        return;
+    }
+    double* out = part_row(part, blockIdx.y);
+    static_assert(MAX_HIST == 15, "one case per history depth");
+    switch (coef[R.id].nh_dots) {  // 0 <= nh_dots <= nhMax <= MAX_HIST (checked at creation)
+#define PHI_CASE(k) \
+    case k: phi_row<k>(R, scal, fuse, redn, out); break;
+        PHI_CASE(0) PHI_CASE(1) PHI_CASE(2) PHI_CASE(3) PHI_CASE(4) PHI_CASE(5) PHI_CASE(6) PHI_CASE(7)
+        PHI_CASE(8) PHI_CASE(9) PHI_CASE(10) PHI_CASE(11) PHI_CASE(12) PHI_CASE(13) PHI_CASE(14) PHI_CASE(15)
+#undef PHI_CASE
+        default: break;
     }
 }
+
 
 // src/anderson.cc:35-55: DD[0][m] = <Dh0, Dh_m>, Dh0 = phi+ - 1 (m = 0 uses Dh0 itself).
 // One instantiation per history depth NH: a point's NH history loads are unconditional and
@@ -242,9 +291,11 @@ __global__ __launch_bounds__(256) void k_am_dots(const Rep* __restrict__ reps, c
 // src/anderson.cc:57-89: w+ += lambda Dh0 + sum_n C_n((wh_n + lambda Dh_n) - (wh0 + lambda Dh0)),
 // then (Dh0, wh0) become history slot `slot`.
 // One instantiation per history depth NH, so a point's 2 NH history loads are all issued
-// before the first update term; the terms are added in the order n = 1 .. NH.
+// before the first update term; the terms are added in the order n = 1 .. NH. With
+// with_h the pass also forms the Boltzmann factors of the new fields, with k_prepare_h's
+// expressions (src/diblock.cc:64-72), so the next evaluation can skip that pass.
 template <int NH>
-__device__ __forceinline__ void am_update_row(const Rep& R, const Coef& c) {
+__device__ __forceinline__ void am_update_row(const Rep& R, const Coef& c, int with_h) {
     const double lambda = c.lambda;
     const int slot = c.slot;
     const long M = R.M;
@@ -254,6 +305,7 @@ __device__ __forceinline__ void am_update_row(const Rep& R, const Coef& c) {
     FOR_POINTS(r, R) {
         const double d0 = R.w[3 * M + r] - 1.0;
         const double w0 = R.w[M + r];
+        const double wm = with_h ? R.w[r] : 0.0;
         double vw[NH + 1], vd[NH + 1];
 #pragma unroll
         for (int n = 1; n <= NH; n++) {
@@ -267,15 +319,20 @@ __device__ __forceinline__ void am_update_row(const Rep& R, const Coef& c) {
         R.w[M + r] = wn;
         R.Dh[(long)slot * M + r] = d0;
         R.wh[(long)slot * M + r] = w0;
+        if (with_h) {
+            R.h[r] = exp(-(wm + wn) / R.N);
+            R.h[r + M] = exp(-(-wm + wn) / R.N);
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void k_am_update(const Rep* __restrict__ reps, const Coef* __restrict__ coef) {
+__global__ __launch_bounds__(256) void k_am_update(const Rep* __restrict__ reps, const Coef* __restrict__ coef,
+                                                   int with_h) {
     REPLICA(R);
     const Coef& c = coef[R.id];
     switch (c.nh) {  // 0 <= nh <= nhMax <= MAX_HIST (checked at creation)
 #define AM_UPDATE_CASE(k) \
-    case k: am_update_row<k>(R, c); break;
+    case k: am_update_row<k>(R, c, with_h); break;
         AM_UPDATE_CASE(0) AM_UPDATE_CASE(1) AM_UPDATE_CASE(2) AM_UPDATE_CASE(3) AM_UPDATE_CASE(4)
         AM_UPDATE_CASE(5) AM_UPDATE_CASE(6) AM_UPDATE_CASE(7) AM_UPDATE_CASE(8) AM_UPDATE_CASE(9)
         AM_UPDATE_CASE(10) AM_UPDATE_CASE(11) AM_UPDATE_CASE(12) AM_UPDATE_CASE(13) AM_UPDATE_CASE(14)
@@ -438,14 +495,15 @@ hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncom
                          int scal_off, hipStream_t st) {
     LAUNCH(k_reduce, dim3(nact), reps, part, ncomp, div_by_M, scal, scal_off);
 }
-hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, int fuse, hipStream_t st) {
-    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal, fuse);
+hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, int fuse, const Coef* coef, double* part,
+                      hipStream_t st) {
+    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal, fuse, coef, part);
 }
 hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st) {
     LAUNCH(k_am_dots, dim3(nb_max, nact), reps, coef, part);
 }
-hipError_t launch_am_update(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st) {
-    LAUNCH(k_am_update, dim3(nb_max, nact), reps, coef);
+hipError_t launch_am_update(const Rep* reps, int nact, int nb_max, const Coef* coef, int with_h, hipStream_t st) {
+    LAUNCH(k_am_update, dim3(nb_max, nact), reps, coef, with_h);
 }
 hipError_t launch_langevin(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st) {
     LAUNCH(k_langevin, dim3(nb_max, nact), reps, coef);

@@ -145,7 +145,10 @@ int line_contour_step(lfts_ctx* c, const double* in, double* out, const double* 
 int set_active(lfts_batch* b, const std::vector<int>& ids);
 // store_all = false lets slab-path replicas fuse the density sums into the chain, which
 // keeps only pairs 1..ceil(N/2) and N; store_all = true keeps every pair.
-int calc_concs_async(lfts_batch* b, bool store_all = false);
+// h_ready: the active members' Boltzmann factors already match their fields (written by
+// the Anderson update just before), so the prepare_h pass is skipped. dots: the phi pass
+// also writes the Gram row partials of each member's next Anderson iteration.
+int calc_concs_async(lfts_batch* b, bool store_all = false, bool h_ready = false, bool dots = false);
 int slab_variant(lfts_batch* b, int edge, int cnt, const lfts::Rep* d_reps, const lfts::Rep* h_reps,
                  const double2* tw, hipStream_t s, int* variant);
 int read_scal(lfts_batch* b);
