@@ -352,11 +352,12 @@ int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep*
 
 // One propagator evaluation (src/diblock.cc:30-60) of every active replica; Q of
 // replica id lands in d_scal[id][0].
-int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, bool dots) {
+int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots) {
     const int nact = (int)b->act.size();
     if (nact == 0) return LFTS_OK;
     if (b->ctx.size() == 1 && b->ctx[0]->ck_stride) {  // checkpointed storage: its own phi pass
         if (int rc = ckpt_calc_concs(b->ctx[0])) return rc;
+        if (dots == DOTS_ONE) return bfail(b, LFTS_EINVAL, "by-value Gram row with checkpointed storage");
         if (dots) BHIPCHK(b, lfts::launch_am_dots(b->d_ract, nact, b->nb_max, b->d_coef, b->d_part, b->st));
         return LFTS_OK;
     }
@@ -418,8 +419,9 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, bool dots) {
     if (rec) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
     BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_part, b->st));
     BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, 1, 1, b->d_scal, 0, b->st));
-    BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, fuse, dots ? b->d_coef : nullptr, b->d_part,
-                                b->st));
+    const int nh_one = (dots == DOTS_ONE) ? b->h_coef[b->act[0]].nh_dots : -1;
+    BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, fuse, dots == DOTS_COEF ? b->d_coef : nullptr,
+                                nh_one, b->d_part, b->st));
     if (rec) {
         BHIPCHK(b, hipEventRecord(ev.pe1, b->st));
         ev.npasses = npasses;
@@ -468,9 +470,13 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
     // pinned coefficients are rewritten only after read_scal has synchronised, so the
     // next evaluation is enqueued while the update still runs, and the descriptors are
     // uploaded again only when the active set changes.
+    // One member (not checkpointed) passes its coefficients as kernel arguments instead:
+    // no upload, and no copy for the update to wait on.
+    const bool byval = n == 1 && !b->ctx[0]->ck_stride;
     if (!active.empty()) {
         for (int id : active) b->h_coef[id].nh_dots = 0;
-        if (int rc = upload_coef(b)) return rc;
+        if (!byval)
+            if (int rc = upload_coef(b)) return rc;
     }
     std::vector<int> cur;
     // From the second iteration on, every member still active was in the previous
@@ -482,7 +488,7 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             cur = active;
         }
         // the evaluation's phi pass also writes the Gram row partials (dots)
-        if (int rc = calc_concs_async(b, false, h_ready, true)) return rc;
+        if (int rc = calc_concs_async(b, false, h_ready, byval ? DOTS_ONE : DOTS_COEF)) return rc;
         int nh_max = 0;
         for (int id : active) {
             const lfts_ctx* c = b->ctx[id];
@@ -518,8 +524,12 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             }
             co.slot = 1 + (k[id] - 1) % c->nhMax;
         }
-        if (int rc = upload_coef(b)) return rc;
-        BHIPCHK(b, lfts::launch_am_update(b->d_ract, nact, b->nb_max, b->d_coef, 1, b->st));
+        if (byval) {
+            BHIPCHK(b, lfts::launch_am_update_one(b->d_ract, b->nb_max, b->h_coef[active[0]], 1, b->st));
+        } else {
+            if (int rc = upload_coef(b)) return rc;
+            BHIPCHK(b, lfts::launch_am_update(b->d_ract, nact, b->nb_max, b->d_coef, 1, b->st));
+        }
         h_ready = true;
         std::vector<int> still;
         for (int id : active) {

@@ -227,18 +227,21 @@ __device__ __forceinline__ void phi_row(const Rep& R, const double* scal, int fu
 }
 
 // With coef, also the Gram row partials of each member's next Anderson iteration
-// (history depth coef[id].nh_dots) into part, as k_am_dots writes them.
+// (history depth coef[id].nh_dots) into part, as k_am_dots writes them; nh_one >= 0
+// gives that depth directly (one member, no coefficient array).
 __global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const double* __restrict__ scal,
-                                             int fuse, const Coef* __restrict__ coef, double* __restrict__ part) {
+                                             int fuse, const Coef* __restrict__ coef, int nh_one,
+                                             double* __restrict__ part) {
     __shared__ double redn[4 * (MAX_HIST + 1)];
     REPLICA(R);
-    if (!coef) {
+    const int nh = nh_one >= 0 ? nh_one : (coef ? coef[R.id].nh_dots : -1);
+    if (nh < 0) {
         phi_row<-1>(R, scal, fuse, redn, nullptr);
         return;
     }
     double* out = part_row(part, blockIdx.y);
     static_assert(MAX_HIST == 15, "one case per history depth");
-    switch (coef[R.id].nh_dots) {  // 0 <= nh_dots <= nhMax <= MAX_HIST (checked at creation)
+    switch (nh) {  // nh_dots <= nhMax <= MAX_HIST (checked at creation)
 #define PHI_CASE(k) \
     case k: phi_row<k>(R, scal, fuse, redn, out); break;
         PHI_CASE(0) PHI_CASE(1) PHI_CASE(2) PHI_CASE(3) PHI_CASE(4) PHI_CASE(5) PHI_CASE(6) PHI_CASE(7)
@@ -326,10 +329,7 @@ __device__ __forceinline__ void am_update_row(const Rep& R, const Coef& c, int w
     }
 }
 
-__global__ __launch_bounds__(256) void k_am_update(const Rep* __restrict__ reps, const Coef* __restrict__ coef,
-                                                   int with_h) {
-    REPLICA(R);
-    const Coef& c = coef[R.id];
+__device__ __forceinline__ void am_update_any(const Rep& R, const Coef& c, int with_h) {
     switch (c.nh) {  // 0 <= nh <= nhMax <= MAX_HIST (checked at creation)
 #define AM_UPDATE_CASE(k) \
     case k: am_update_row<k>(R, c, with_h); break;
@@ -340,6 +340,18 @@ __global__ __launch_bounds__(256) void k_am_update(const Rep* __restrict__ reps,
 #undef AM_UPDATE_CASE
         default: break;
     }
+}
+
+__global__ __launch_bounds__(256) void k_am_update(const Rep* __restrict__ reps, const Coef* __restrict__ coef,
+                                                   int with_h) {
+    REPLICA(R);
+    am_update_any(R, coef[R.id], with_h);
+}
+
+// One member, its coefficients passed by value (no coefficient array to upload).
+__global__ __launch_bounds__(256) void k_am_update_one(const Rep* __restrict__ reps, const Coef c, int with_h) {
+    REPLICA(R);
+    am_update_any(R, c, with_h);
 }
 
 // src/langevin.cc:31: W- += -(phi- + 2 W-/XbN) dt + 0.5 (eta_prev + eta_new); with keep_new the
@@ -495,15 +507,20 @@ hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncom
                          int scal_off, hipStream_t st) {
     LAUNCH(k_reduce, dim3(nact), reps, part, ncomp, div_by_M, scal, scal_off);
 }
-hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, int fuse, const Coef* coef, double* part,
-                      hipStream_t st) {
-    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal, fuse, coef, part);
+hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, int fuse, const Coef* coef, int nh_one,
+                      double* part, hipStream_t st) {
+    if (nh_one > MAX_HIST || (nh_one >= 0 && nact != 1)) return hipErrorInvalidValue;
+    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal, fuse, coef, nh_one, part);
 }
 hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st) {
     LAUNCH(k_am_dots, dim3(nb_max, nact), reps, coef, part);
 }
 hipError_t launch_am_update(const Rep* reps, int nact, int nb_max, const Coef* coef, int with_h, hipStream_t st) {
     LAUNCH(k_am_update, dim3(nb_max, nact), reps, coef, with_h);
+}
+hipError_t launch_am_update_one(const Rep* reps, int nb_max, const Coef& c, int with_h, hipStream_t st) {
+    if (c.nh < 0 || c.nh > MAX_HIST) return hipErrorInvalidValue;
+    LAUNCH(k_am_update_one, dim3(nb_max, 1), reps, c, with_h);
 }
 hipError_t launch_langevin(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st) {
     LAUNCH(k_langevin, dim3(nb_max, nact), reps, coef);

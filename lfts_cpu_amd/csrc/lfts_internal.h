@@ -215,12 +215,14 @@ hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncom
 // phi from the fused sums (replicas with fused_phi(R, fuse)) or from the stored pairs,
 // both in the same order of operations
 // coef (may be null): also write each member's next Anderson Gram row partials into part,
-// as launch_am_dots would
-hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, int fuse, const Coef* coef, double* part,
-                      hipStream_t st);
+// as launch_am_dots would; nh_one >= 0 (one member only) gives the depth instead of coef
+hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, int fuse, const Coef* coef, int nh_one,
+                      double* part, hipStream_t st);
 hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st);
 // with_h: also write h = exp(-(+-W- + w+)/N) of the updated fields, as launch_prepare_h would
 hipError_t launch_am_update(const Rep* reps, int nact, int nb_max, const Coef* coef, int with_h, hipStream_t st);
+// the same for one member with its coefficients as a kernel argument
+hipError_t launch_am_update_one(const Rep* reps, int nb_max, const Coef& c, int with_h, hipStream_t st);
 hipError_t launch_langevin(const Rep* reps, int nact, int nb_max, const Coef* coef, hipStream_t st);
 hipError_t launch_h_sums(const Rep* reps, int nact, int nb_max, double* part, hipStream_t st);
 

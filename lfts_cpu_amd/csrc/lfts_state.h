@@ -147,8 +147,11 @@ int set_active(lfts_batch* b, const std::vector<int>& ids);
 // keeps only pairs 1..ceil(N/2) and N; store_all = true keeps every pair.
 // h_ready: the active members' Boltzmann factors already match their fields (written by
 // the Anderson update just before), so the prepare_h pass is skipped. dots: the phi pass
-// also writes the Gram row partials of each member's next Anderson iteration.
-int calc_concs_async(lfts_batch* b, bool store_all = false, bool h_ready = false, bool dots = false);
+// also writes the Gram row partials of each member's next Anderson iteration, with the
+// history depths of the uploaded coefficients (DOTS_COEF) or, for one member, that of
+// the pinned host coefficients passed as a kernel argument (DOTS_ONE).
+enum { DOTS_NONE = 0, DOTS_COEF = 1, DOTS_ONE = 2 };
+int calc_concs_async(lfts_batch* b, bool store_all = false, bool h_ready = false, int dots = DOTS_NONE);
 int slab_variant(lfts_batch* b, int edge, int cnt, const lfts::Rep* d_reps, const lfts::Rep* h_reps,
                  const double2* tw, hipStream_t s, int* variant);
 int read_scal(lfts_batch* b);
