@@ -417,11 +417,10 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots) {
             BHIPCHK(b, hipStreamWaitEvent(b->st, b->ev_join[g], 0));
         }
     if (rec) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
-    BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_part, b->st));
-    BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, 1, 1, b->d_scal, 0, b->st));
+    BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_qpart, b->st));
     const int nh_one = (dots == DOTS_ONE) ? b->h_coef[b->act[0]].nh_dots : -1;
-    BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, fuse, dots == DOTS_COEF ? b->d_coef : nullptr,
-                                nh_one, b->d_part, b->st));
+    BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, b->d_qpart, fuse,
+                                dots == DOTS_COEF ? b->d_coef : nullptr, nh_one, b->d_part, b->st));
     if (rec) {
         BHIPCHK(b, hipEventRecord(ev.pe1, b->st));
         ev.npasses = npasses;
@@ -719,6 +718,7 @@ int batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
     std::memset(b->h_coef, 0, sizeof(Coef) * n);
     BCREATE(hipMalloc(&b->d_part, sizeof(double) * lfts::PSTRIDE * lfts::MAX_NB * n));
     BCREATE(hipMalloc(&b->d_scal, sizeof(double) * lfts::SSTRIDE * n));
+    BCREATE(hipMalloc(&b->d_qpart, sizeof(double) * lfts::MAX_NB * n));
     BCREATE(hipHostMalloc(&b->h_scal, sizeof(double) * lfts::SSTRIDE * n, hipHostMallocDefault));
 #undef BCREATE
     *out = b;
@@ -1104,7 +1104,8 @@ int lfts_batch_destroy(lfts_batch* b) {
         if (b->ev_join[g]) (void)hipEventDestroy(b->ev_join[g]);
     }
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-    for (void* p : {(void*)b->d_ract, (void*)b->d_coef, (void*)b->d_part, (void*)b->d_scal}) (void)hipFree(p);
+    for (void* p : {(void*)b->d_ract, (void*)b->d_coef, (void*)b->d_part, (void*)b->d_scal, (void*)b->d_qpart})
+        (void)hipFree(p);
     for (void* p : {(void*)b->h_ract, (void*)b->h_coef, (void*)b->h_scal})
         if (p) (void)hipHostFree(p);
     if (b->st) (void)hipStreamDestroy(b->st);

@@ -34,7 +34,7 @@ __device__ double block_sum(double v, double* red) {
 // block_sum of nc components at once (component c's result in out[c], written by thread
 // c): each component takes exactly block_sum's operations, with one barrier for all.
 template <int NC>
-__device__ void block_sums(double (&v)[NC], int nc, double* red, double* out) {
+__device__ __forceinline__ void block_sums(double (&v)[NC], int nc, double* red, double* out) {
 #pragma unroll
     for (int c = 0; c < NC; c++)
         if (c < nc) v[c] = wave_sum(v[c]);
@@ -79,14 +79,38 @@ __global__ __launch_bounds__(256) void k_prepare_h(const Rep* __restrict__ reps)
 }
 
 // partial sums of q_N (src/diblock.cc:46-48)
-__global__ __launch_bounds__(256) void k_q_sum(const Rep* __restrict__ reps, double* __restrict__ part) {
+// into qpart[slot][workgroup]; k_phi forms Q from them
+__global__ __launch_bounds__(256) void k_q_sum(const Rep* __restrict__ reps, double* __restrict__ qpart) {
     __shared__ double red[8];
     REPLICA(R);
     const double* qN = pair_ptr(R, R.N);
     double s = 0.0;
     FOR_POINTS(r, R) s += qN[r];
     s = block_sum(s, red);
-    if (threadIdx.x == 0) part_row(part, blockIdx.y)[0] = s;
+    if (threadIdx.x == 0) qpart[(long)blockIdx.y * MAX_NB + blockIdx.x] = s;
+}
+
+// Q = (sum of the replica's k_q_sum partials) / M, in every workgroup that asks, with
+// k_reduce's operations in k_reduce's order (thread t adds rows t, t + 256, ... < nb, then
+// the wavefront and LDS sums), so every workgroup gets the same bits. All of a thread's
+// rows are loaded first; indexes from nb up are clamped in bounds and never added.
+__device__ __forceinline__ double q_from_partials(const Rep& R, const double* qpart, double* red, double* res) {
+    constexpr int ROWS = MAX_NB / 256;  // launched with 256 threads
+    const double* p = qpart + (long)blockIdx.y * MAX_NB;
+    const int nb = R.nb;
+    double v[ROWS];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+        const int b = (int)threadIdx.x + 256 * k;
+        v[k] = p[(b < nb) ? b : nb - 1];
+    }
+    double s[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < ROWS; k++)
+        if ((int)threadIdx.x + 256 * k < nb) s[0] += v[k];
+    block_sums<1>(s, 1, red, res);
+    __syncthreads();
+    return res[0] / R.M;
 }
 
 // scal[rid][off + c] = sum over the replica's partials (divided by M when asked), all
@@ -141,8 +165,7 @@ __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, co
 // (Dh0 = phi+ - 1, src/anderson.cc:35-55) over the same points in the same order as
 // k_am_dots, so the partial rows are identical and the separate pass is saved.
 template <int NH>
-__device__ __forceinline__ void phi_row(const Rep& R, const double* scal, int fuse, double* redn, double* out) {
-    const double Q = scal[(long)R.id * SSTRIDE];
+__device__ __forceinline__ void phi_row(const Rep& R, double Q, int fuse, double* redn, double* out) {
     const long M = R.M;
     const int N = R.N, NA = R.NA, i0 = (N + 1) / 2;
     const bool fused = fused_phi(R, fuse);
@@ -229,21 +252,26 @@ __device__ __forceinline__ void phi_row(const Rep& R, const double* scal, int fu
 // With coef, also the Gram row partials of each member's next Anderson iteration
 // (history depth coef[id].nh_dots) into part, as k_am_dots writes them; nh_one >= 0
 // gives that depth directly (one member, no coefficient array).
-__global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, const double* __restrict__ scal,
-                                             int fuse, const Coef* __restrict__ coef, int nh_one,
-                                             double* __restrict__ part) {
+// Q comes from the k_q_sum partials (q_from_partials); workgroup 0 also stores it in
+// scal[id][0] for the host.
+__global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, double* __restrict__ scal,
+                                             const double* __restrict__ qpart, int fuse,
+                                             const Coef* __restrict__ coef, int nh_one, double* __restrict__ part) {
     __shared__ double redn[4 * (MAX_HIST + 1)];
+    __shared__ double qred[4], qres[1];
     REPLICA(R);
+    const double Q = q_from_partials(R, qpart, qred, qres);
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[(long)R.id * SSTRIDE] = Q;
     const int nh = nh_one >= 0 ? nh_one : (coef ? coef[R.id].nh_dots : -1);
     if (nh < 0) {
-        phi_row<-1>(R, scal, fuse, redn, nullptr);
+        phi_row<-1>(R, Q, fuse, redn, nullptr);
         return;
     }
     double* out = part_row(part, blockIdx.y);
     static_assert(MAX_HIST == 15, "one case per history depth");
     switch (nh) {  // nh_dots <= nhMax <= MAX_HIST (checked at creation)
 #define PHI_CASE(k) \
-    case k: phi_row<k>(R, scal, fuse, redn, out); break;
+    case k: phi_row<k>(R, Q, fuse, redn, out); break;
         PHI_CASE(0) PHI_CASE(1) PHI_CASE(2) PHI_CASE(3) PHI_CASE(4) PHI_CASE(5) PHI_CASE(6) PHI_CASE(7)
         PHI_CASE(8) PHI_CASE(9) PHI_CASE(10) PHI_CASE(11) PHI_CASE(12) PHI_CASE(13) PHI_CASE(14) PHI_CASE(15)
 #undef PHI_CASE
@@ -500,17 +528,17 @@ __global__ void k_am_advance(AmState* __restrict__ s) {
 hipError_t launch_prepare_h(const Rep* reps, int nact, int nb_max, hipStream_t st) {
     LAUNCH(k_prepare_h, dim3(nb_max, nact), reps);
 }
-hipError_t launch_q_sum(const Rep* reps, int nact, int nb_max, double* part, hipStream_t st) {
-    LAUNCH(k_q_sum, dim3(nb_max, nact), reps, part);
+hipError_t launch_q_sum(const Rep* reps, int nact, int nb_max, double* qpart, hipStream_t st) {
+    LAUNCH(k_q_sum, dim3(nb_max, nact), reps, qpart);
 }
 hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncomp, int div_by_M, double* scal,
                          int scal_off, hipStream_t st) {
     LAUNCH(k_reduce, dim3(nact), reps, part, ncomp, div_by_M, scal, scal_off);
 }
-hipError_t launch_phi(const Rep* reps, int nact, int nb_max, const double* scal, int fuse, const Coef* coef, int nh_one,
-                      double* part, hipStream_t st) {
+hipError_t launch_phi(const Rep* reps, int nact, int nb_max, double* scal, const double* qpart, int fuse,
+                      const Coef* coef, int nh_one, double* part, hipStream_t st) {
     if (nh_one > MAX_HIST || (nh_one >= 0 && nact != 1)) return hipErrorInvalidValue;
-    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal, fuse, coef, nh_one, part);
+    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal, qpart, fuse, coef, nh_one, part);
 }
 hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st) {
     LAUNCH(k_am_dots, dim3(nb_max, nact), reps, coef, part);

@@ -95,6 +95,7 @@ struct lfts_batch {
     lfts::Coef* h_coef = nullptr;  // pinned
     double* d_part = nullptr;
     double* d_scal = nullptr;
+    double* d_qpart = nullptr;  // k_q_sum partials, MAX_NB per launch slot
     double* h_scal = nullptr;  // pinned
     // the active set as launched: slab replicas sorted by edge, then line-path ones
     std::vector<int> act;
