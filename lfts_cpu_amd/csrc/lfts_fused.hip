@@ -523,8 +523,8 @@ __device__ __forceinline__ int stage2_dft(int j, int u) {
 
 // Twiddles w_n^{jk} (k < R1) and kernel values g(jU + u + R1 f) of the line lane j a
 // thread serves: held in registers (loaded once per pass) or read from the workgroup's
-// LDS copy of the axis tables when used (fewer registers; 48-point lines need that to
-// run two waves per SIMD).
+// LDS copy of the axis tables when used (fewer registers, so more waves per SIMD where a
+// pass is register-bound).
 template <int R1, int R2>
 struct RegTab {
     static constexpr int U = stage2_per_thread<R1, R2>();
@@ -884,7 +884,7 @@ hipError_t dispatch_edge(int n, int v, const Rep* reps, int nact, int i, int mod
     else if (n == 96) dispatch<96, 12, 8>(1, reps, nact, i, mode, io, tw, st);
     else if (n == 40) dispatch<40, 5, 8>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 16) dispatch<16, 4, 4>(v, reps, nact, i, mode, io, tw, st);
-    else if (n == 48) dispatch<48, 12, 4>(v, reps, nact, i, mode, io, tw, st);
+    else if (n == 48) dispatch<48, 6, 8>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 32) dispatch<32, 8, 4>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 80) dispatch<80, 10, 8>(1, reps, nact, i, mode, io, tw, st);
     else if (n == 72) dispatch<72, 9, 8>(1, reps, nact, i, mode, io, tw, st);
