@@ -1029,9 +1029,9 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
     switch (a.n) {
         case 16: return axis_launch<16, 4, 4>(a, st);
         case 24: return axis_launch<24, 3, 8>(a, st);
-        case 32: return axis_launch<32, 8, 4>(a, st);
+        case 32: return axis_launch<32, 4, 8>(a, st);
         case 40: return axis_launch<40, 5, 8>(a, st);
-        case 48: return axis_launch<48, 12, 4, 3, 16>(a, st);
+        case 48: return axis_launch<48, 6, 8, 3, 16>(a, st);
         case 56: return axis_launch<56, 7, 8>(a, st);
         case 64: return axis_launch<64, 8, 8>(a, st);
         case 72: return axis_launch<72, 9, 8>(a, st);

@@ -285,7 +285,9 @@ def test_fused_slab_path_against_oracle_and_generic_path(gpu, monkeypatch, n, N,
                                        ((6, 8, 168), (2.0, 2.2, 6.8), 5, 3),
                                        ((4, 6, 250), (1.8, 2.0, 8.2), 5, 2),
                                        ((4, 8, 216), (1.8, 2.2, 7.6), 5, 4),
-                                       ((252, 6, 4), (8.3, 2.0, 1.8), 4, 2)])
+                                       ((252, 6, 4), (8.3, 2.0, 1.8), 4, 2),
+                                       ((40, 24, 32), (3.4, 2.9, 3.1), 6, 2),    # z lines of 32 and 48 points
+                                       ((12, 20, 48), (2.4, 2.6, 3.7), 6, 3)])
 def test_axis_kernel_against_generic_line_kernel(gpu, monkeypatch, m, Lb, N, NA):
     """Line-path meshes whose axes have slab-edge lengths run the register-FFT axis kernel;
     LFTS_PATH=generic runs the generic line kernel on the same input. Pairs, lnQ and phi
