@@ -886,8 +886,8 @@ hipError_t dispatch_edge(int n, int v, const Rep* reps, int nact, int i, int mod
     else if (n == 16) dispatch<16, 4, 4>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 48) dispatch<48, 6, 8>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 32) dispatch<32, 4, 8>(v, reps, nact, i, mode, io, tw, st);
-    else if (n == 80) dispatch<80, 10, 8>(1, reps, nact, i, mode, io, tw, st);
-    else if (n == 72) dispatch<72, 9, 8>(1, reps, nact, i, mode, io, tw, st);
+    else if (n == 80) dispatch<80, 8, 10>(1, reps, nact, i, mode, io, tw, st);
+    else if (n == 72) dispatch<72, 8, 9>(1, reps, nact, i, mode, io, tw, st);
     else if (n == 56) dispatch<56, 7, 8>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 24) dispatch<24, 3, 8>(v, reps, nact, i, mode, io, tw, st);
     else if (n == 112) dispatch<112, 14, 8>(1, reps, nact, i, mode, io, tw, st);
