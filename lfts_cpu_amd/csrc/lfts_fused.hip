@@ -1036,8 +1036,8 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 64: return axis_launch<64, 8, 8>(a, st);
         case 72: return axis_launch<72, 8, 9, 9, 8>(a, st);
         case 80: return axis_launch<80, 8, 10, 5, 16>(a, st);
-        case 96: return axis_launch<96, 12, 8, 6, 16>(a, st);
-        case 112: return axis_launch<112, 14, 8, 7, 16>(a, st);
+        case 96: return axis_launch<96, 8, 12, 6, 16>(a, st);
+        case 112: return axis_launch<112, 8, 14, 7, 16>(a, st);
         case 120: return axis_launch<120, 15, 8>(a, st);
         case 128: return axis_launch<128, 16, 8, 8, 16>(a, st);
         // lengths the slab pass does not serve (a plane would not fit in LDS, or no plane
@@ -1045,7 +1045,7 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 20: return axis_launch<20, 4, 5, 5, 4>(a, st);
         case 36: return axis_launch<36, 6, 6, 9, 4>(a, st);
         case 60: return axis_launch<60, 6, 10, 15, 4>(a, st);
-        case 144: return axis_launch<144, 18, 8, 9, 16>(a, st);
+        case 144: return axis_launch<144, 12, 12, 9, 16>(a, st);
         case 160: return axis_launch<160, 20, 8, 10, 16>(a, st);
         case 192: return axis_launch<192, 24, 8, 12, 16>(a, st);
         case 200: return axis_launch<200, 25, 8>(a, st);
@@ -1056,13 +1056,13 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         // further lengths with factors 2, 3, 5 and 7 (R1 x R2 from the register DFTs)
         case 18: return axis_launch<18, 3, 6>(a, st);
         case 28: return axis_launch<28, 4, 7, 7, 4>(a, st);
-        case 42: return axis_launch<42, 7, 6>(a, st);
-        case 54: return axis_launch<54, 9, 6>(a, st);
+        case 42: return axis_launch<42, 6, 7, 7, 6>(a, st);
+        case 54: return axis_launch<54, 6, 9, 9, 6>(a, st);
         case 70: return axis_launch<70, 7, 10>(a, st);
-        case 84: return axis_launch<84, 12, 7>(a, st);
+        case 84: return axis_launch<84, 7, 12, 12, 7>(a, st);
         case 90: return axis_launch<90, 9, 10>(a, st);
         case 98: return axis_launch<98, 14, 7>(a, st);
-        case 108: return axis_launch<108, 12, 9>(a, st);
+        case 108: return axis_launch<108, 9, 12, 12, 9>(a, st);
         case 126: return axis_launch<126, 14, 9>(a, st);
         case 140: return axis_launch<140, 14, 10>(a, st);
         case 150: return axis_launch<150, 15, 10>(a, st);
