@@ -1046,9 +1046,9 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 36: return axis_launch<36, 6, 6, 9, 4>(a, st);
         case 60: return axis_launch<60, 6, 10, 15, 4>(a, st);
         case 144: return axis_launch<144, 12, 12, 9, 16>(a, st);
-        case 160: return axis_launch<160, 20, 8, 10, 16>(a, st);
+        case 160: return axis_launch<160, 10, 16, 10, 16>(a, st);
         case 192: return axis_launch<192, 24, 8, 12, 16>(a, st);
-        case 200: return axis_launch<200, 25, 8>(a, st);
+        case 200: return axis_launch<200, 10, 20, 25, 8>(a, st);
         case 256: return axis_launch<256, 16, 16>(a, st);
         case 30: return axis_launch<30, 5, 6, 10, 3>(a, st);
         case 50: return axis_launch<50, 10, 5>(a, st);
@@ -1063,18 +1063,18 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 90: return axis_launch<90, 9, 10>(a, st);
         case 98: return axis_launch<98, 14, 7>(a, st);
         case 108: return axis_launch<108, 9, 12, 12, 9>(a, st);
-        case 126: return axis_launch<126, 14, 9>(a, st);
-        case 140: return axis_launch<140, 14, 10>(a, st);
-        case 150: return axis_launch<150, 15, 10>(a, st);
+        case 126: return axis_launch<126, 9, 14, 14, 9>(a, st);
+        case 140: return axis_launch<140, 10, 14, 14, 10>(a, st);
+        case 150: return axis_launch<150, 10, 15, 15, 10>(a, st);
         case 162: return axis_launch<162, 18, 9>(a, st);
-        case 168: return axis_launch<168, 14, 12>(a, st);
+        case 168: return axis_launch<168, 12, 14, 14, 12>(a, st);
         case 180: return axis_launch<180, 15, 12>(a, st);
         case 196: return axis_launch<196, 14, 14>(a, st);
-        case 210: return axis_launch<210, 15, 14>(a, st);
+        case 210: return axis_launch<210, 14, 15, 15, 14>(a, st);
         case 216: return axis_launch<216, 18, 12>(a, st);
         case 224: return axis_launch<224, 14, 16>(a, st);
         case 240: return axis_launch<240, 15, 16>(a, st);
-        case 250: return axis_launch<250, 25, 10>(a, st);
+        case 250: return axis_launch<250, 10, 25, 25, 10>(a, st);
         case 252: return axis_launch<252, 18, 14>(a, st);
         default: return hipErrorNotSupported;
     }
