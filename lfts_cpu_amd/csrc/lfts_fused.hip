@@ -1034,7 +1034,7 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 48: return axis_launch<48, 6, 8, 3, 16>(a, st);
         case 56: return axis_launch<56, 7, 8>(a, st);
         case 64: return axis_launch<64, 8, 8>(a, st);
-        case 72: return axis_launch<72, 8, 9, 9, 8>(a, st);
+        case 72: return axis_launch<72, 8, 9, 8, 9>(a, st);
         case 80: return axis_launch<80, 8, 10, 5, 16>(a, st);
         case 96: return axis_launch<96, 8, 12, 6, 16>(a, st);
         case 112: return axis_launch<112, 8, 14, 7, 16>(a, st);
@@ -1042,39 +1042,39 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 128: return axis_launch<128, 16, 8, 8, 16>(a, st);
         // lengths the slab pass does not serve (a plane would not fit in LDS, or no plane
         // factorisation): the axis kernel keeps only a 16-line tile in LDS
-        case 20: return axis_launch<20, 4, 5, 5, 4>(a, st);
-        case 36: return axis_launch<36, 6, 6, 9, 4>(a, st);
-        case 60: return axis_launch<60, 6, 10, 15, 4>(a, st);
+        case 20: return axis_launch<20, 4, 5, 4, 5>(a, st);
+        case 36: return axis_launch<36, 6, 6, 6, 6>(a, st);
+        case 60: return axis_launch<60, 6, 10, 6, 10>(a, st);
         case 144: return axis_launch<144, 12, 12, 9, 16>(a, st);
         case 160: return axis_launch<160, 10, 16, 10, 16>(a, st);
         case 192: return axis_launch<192, 24, 8, 12, 16>(a, st);
-        case 200: return axis_launch<200, 10, 20, 25, 8>(a, st);
+        case 200: return axis_launch<200, 10, 20, 10, 20>(a, st);
         case 256: return axis_launch<256, 16, 16>(a, st);
-        case 30: return axis_launch<30, 5, 6, 10, 3>(a, st);
+        case 30: return axis_launch<30, 5, 6, 5, 6>(a, st);
         case 50: return axis_launch<50, 10, 5>(a, st);
-        case 100: return axis_launch<100, 10, 10, 25, 4>(a, st);
+        case 100: return axis_launch<100, 10, 10, 10, 10>(a, st);
         // further lengths with factors 2, 3, 5 and 7 (R1 x R2 from the register DFTs)
         case 18: return axis_launch<18, 3, 6>(a, st);
-        case 28: return axis_launch<28, 4, 7, 7, 4>(a, st);
-        case 42: return axis_launch<42, 6, 7, 7, 6>(a, st);
-        case 54: return axis_launch<54, 6, 9, 9, 6>(a, st);
+        case 28: return axis_launch<28, 4, 7, 4, 7>(a, st);
+        case 42: return axis_launch<42, 6, 7, 6, 7>(a, st);
+        case 54: return axis_launch<54, 6, 9, 6, 9>(a, st);
         case 70: return axis_launch<70, 7, 10>(a, st);
-        case 84: return axis_launch<84, 7, 12, 12, 7>(a, st);
+        case 84: return axis_launch<84, 7, 12, 7, 12>(a, st);
         case 90: return axis_launch<90, 9, 10>(a, st);
         case 98: return axis_launch<98, 14, 7>(a, st);
-        case 108: return axis_launch<108, 9, 12, 12, 9>(a, st);
-        case 126: return axis_launch<126, 9, 14, 14, 9>(a, st);
-        case 140: return axis_launch<140, 10, 14, 14, 10>(a, st);
-        case 150: return axis_launch<150, 10, 15, 15, 10>(a, st);
+        case 108: return axis_launch<108, 9, 12, 9, 12>(a, st);
+        case 126: return axis_launch<126, 9, 14, 9, 14>(a, st);
+        case 140: return axis_launch<140, 10, 14, 10, 14>(a, st);
+        case 150: return axis_launch<150, 10, 15, 10, 15>(a, st);
         case 162: return axis_launch<162, 18, 9>(a, st);
-        case 168: return axis_launch<168, 12, 14, 14, 12>(a, st);
+        case 168: return axis_launch<168, 12, 14, 12, 14>(a, st);
         case 180: return axis_launch<180, 15, 12>(a, st);
         case 196: return axis_launch<196, 14, 14>(a, st);
-        case 210: return axis_launch<210, 14, 15, 15, 14>(a, st);
+        case 210: return axis_launch<210, 14, 15, 14, 15>(a, st);
         case 216: return axis_launch<216, 18, 12>(a, st);
         case 224: return axis_launch<224, 14, 16>(a, st);
         case 240: return axis_launch<240, 15, 16>(a, st);
-        case 250: return axis_launch<250, 10, 25, 25, 10>(a, st);
+        case 250: return axis_launch<250, 10, 25, 10, 25>(a, st);
         case 252: return axis_launch<252, 18, 14>(a, st);
         default: return hipErrorNotSupported;
     }
