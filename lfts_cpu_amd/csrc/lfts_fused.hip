@@ -1038,7 +1038,7 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 80: return axis_launch<80, 8, 10, 5, 16>(a, st);
         case 96: return axis_launch<96, 8, 12, 6, 16>(a, st);
         case 112: return axis_launch<112, 8, 14, 7, 16>(a, st);
-        case 120: return axis_launch<120, 15, 8>(a, st);
+        case 120: return axis_launch<120, 15, 8, 8, 15>(a, st);
         case 128: return axis_launch<128, 16, 8, 8, 16>(a, st);
         // lengths the slab pass does not serve (a plane would not fit in LDS, or no plane
         // factorisation): the axis kernel keeps only a 16-line tile in LDS
@@ -1068,14 +1068,14 @@ hipError_t launch_axis_conv(const LineArgs& a, hipStream_t st) {
         case 150: return axis_launch<150, 10, 15, 10, 15>(a, st);
         case 162: return axis_launch<162, 18, 9>(a, st);
         case 168: return axis_launch<168, 12, 14, 12, 14>(a, st);
-        case 180: return axis_launch<180, 15, 12>(a, st);
+        case 180: return axis_launch<180, 15, 12, 12, 15>(a, st);
         case 196: return axis_launch<196, 14, 14>(a, st);
         case 210: return axis_launch<210, 14, 15, 14, 15>(a, st);
-        case 216: return axis_launch<216, 18, 12>(a, st);
+        case 216: return axis_launch<216, 18, 12, 12, 18>(a, st);
         case 224: return axis_launch<224, 14, 16>(a, st);
         case 240: return axis_launch<240, 15, 16>(a, st);
         case 250: return axis_launch<250, 10, 25, 10, 25>(a, st);
-        case 252: return axis_launch<252, 18, 14>(a, st);
+        case 252: return axis_launch<252, 18, 14, 14, 18>(a, st);
         default: return hipErrorNotSupported;
     }
 }
