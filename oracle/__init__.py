@@ -55,6 +55,10 @@ def lib():
             "orc_am_destroy": (None, [_vp]),
             "orc_am_mix": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double, _dp]),
             "orc_am_last_err": (ctypes.c_double, [_vp]),
+            "orc_am_set_trace": (None, [_vp, _dp, ctypes.c_int]),
+            "orc_am_trace_count": (ctypes.c_int, [_vp]),
+            "orc_am_set_fault": (None, [_vp, ctypes.c_int]),
+            "orc_sk_set_fault": (None, [_vp, ctypes.c_int]),
             "orc_lu_solve": (None, [ctypes.c_int, _dp, _dp, _dp]),
             "orc_langevin_update": (None, [_dp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
             "orc_normals": (None, [ctypes.c_ulonglong, ctypes.c_double, ctypes.c_int, _dp]),
@@ -170,7 +174,9 @@ class Anderson:
     """anderson restatement (src/anderson.cc)."""
 
     def __init__(self, M: int, nhMax: int = 10):
-        self.h = lib().orc_am_create(int(M), int(nhMax))
+        self.M, self.nhMax = int(M), int(nhMax)
+        self.h = lib().orc_am_create(self.M, self.nhMax)
+        self._trace = None
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -182,6 +188,28 @@ class Anderson:
 
     def last_err(self) -> float:
         return lib().orc_am_last_err(self.h)
+
+    def set_fault(self, fault: int):
+        """Plant a misreading of src/anderson.cc (tests of the pins only; 0 = none)."""
+        lib().orc_am_set_fault(self.h, int(fault))
+
+    def record(self, cap: int):
+        """Record up to cap iterations of the following mix calls (see trace())."""
+        self._trace = np.zeros((cap, 5 + self.nhMax + 3 * self.M))
+        lib().orc_am_set_trace(self.h, _d(self._trace), int(cap))
+
+    def trace(self) -> list[dict]:
+        """Per iteration: k, nh, lambda, err, the history slot written at its end, the
+        coefficients C (slot order), Dh0 = phi+ - 1, wh0 = w+ before and w+ after."""
+        n = lib().orc_am_trace_count(self.h)
+        M, H = self.M, self.nhMax
+        out = []
+        for t in self._trace[:n]:
+            nh = int(t[1])
+            out.append(dict(k=int(t[0]), nh=nh, lam=t[2], err=t[3], slot=int(t[4]), C=t[5:5 + nh].copy(),
+                            Dh0=t[5 + H:5 + H + M].copy(), wh0=t[5 + H + M:5 + H + 2 * M].copy(),
+                            w_new=t[5 + H + 2 * M:5 + H + 3 * M].copy()))
+        return out
 
 
 def lu_solve(A: np.ndarray, b: np.ndarray) -> np.ndarray:
@@ -236,6 +264,10 @@ class StrFunc:
 
     def save(self, fname: str, dp: int = 8):
         lib().orc_sk_save(self.h, fname.encode(), dp)
+
+    def set_fault(self, fault: int):
+        """Plant a misreading of src/strFunc.cc's save (tests of the pins only)."""
+        lib().orc_sk_set_fault(self.h, int(fault))
 
 
 def z_inf(L, m, N: int, Nbar: float, tmax: int = 100) -> float:

@@ -25,11 +25,18 @@
 //   calc_concs     closed forms (w=0; uniform w+; uniform W-), the invariant that
 //                  (1/M) sum_r q_i q+_i / h is the same for every bead i, mean(phi+)=1,
 //                  an independent numpy restatement that uses numpy.fft.rfftn/irfftn
-//   anderson       converged residual below tolerance, fixed point reproduced,
-//                  independent numpy restatement of the iterate sequence, LU vs scipy
+//   anderson       every iteration's update against its least-squares statement
+//                  (numpy.linalg.lstsq over the previous nh iterations' histories,
+//                  which the traced slot writes must hold), the closed forms of the
+//                  first two iterations, the lambda/nh/slot schedules across a ring
+//                  wrap, the loop's stop rule; each of six planted misreadings fails
+//                  one of these; converged residual, fixed point; LU vs scipy
 //   langevin       closed-form update; the noise stream against the C++ standard's
 //                  mt19937_64 known answer and a Python restatement of the polar method
-//   strFunc        single cosine mode (|w_k|^2 = (A M/2)^2), bins vs numpy, weights
+//   strFunc        single cosine mode (|w_k|^2 = (A M/2)^2), bins vs numpy, weights;
+//                  save: shell averages of numpy's full-spectrum |FFT|^2 over exact
+//                  |k| shells (even mz), every empty shell -0.5/chi_b, the odd-mz
+//                  weight quirk; each planted misreading of save fails
 //   getH           closed form for uniform fields
 //   jvp, vjp       central finite differences of calc_concs; <u, J v> = <J^T u, v>
 //   dlnQ/dL        central finite differences of calc_concs over the box edges
@@ -489,7 +496,18 @@ struct orc_anderson {
     std::vector<double> U, V, C;
     std::vector<double> Dh, wh; // (nhMax+1) x M
     double last_err = 0.0;
+    // Test hooks. trace: one record per iteration (layout at am_trace_record) for the
+    // least-squares pin of the iterate sequence. fault: a planted misreading of
+    // src/anderson.cc (1..6, see anderson_mix), used only to show that the pins catch it.
+    double* trace = nullptr;
+    int trace_cap = 0, trace_n = 0;
+    int fault = 0;
 };
+
+// Trace record of one iteration: [k, nh, lambda, err, slot, C[0..nhMax-1], Dh0 (M),
+// wh0 (M), w+ after the update (M)]; slot is the history row the iteration's Dh0/wh0
+// are copied into at its end.
+static size_t am_trace_len(const orc_anderson& a) { return 5 + (size_t)a.nhMax + 3 * (size_t)a.M; }
 
 static int anderson_mix(orc_anderson& a, orc_diblock* dbc, int maxIter, double errTol, double* w) {
     const int M = a.M, H = a.nhMax + 1;
@@ -498,14 +516,19 @@ static int anderson_mix(orc_anderson& a, orc_diblock* dbc, int maxIter, double e
     auto DD = [&](int i, int j) -> double& { return a.DD[(size_t)i * H + j]; };
     auto Dh = [&](int i) { return a.Dh.data() + (size_t)i * M; };
     auto wh = [&](int i) { return a.wh.data() + (size_t)i * M; };
+    // planted faults (tests only): 1 lambda = 1-0.9^(k+1); 2 ring slot 1+k%nhMax;
+    // 3 the DD entries past the slot are not refreshed; 4 U laid out with stride nhMax;
+    // 5 nh capped at nhMax-1; 6 the lambda*Dh_n term dropped from the history sum
+    const int F = a.fault;
     for (k = 1; k < maxIter && err > errTol; k++) {
         diblock_calc_concs(*dbc, w);
         for (int r = 0; r < M; r++) { Dh(0)[r] = w[r + 3 * M] - 1.0; wh(0)[r] = w[r + M]; }
         S1 = 0.0;
         for (int r = 0; r < M; r++) S1 += pow(Dh(0)[r], 2.0);
         err = pow(S1 / M, 0.5);
-        lambda = 1.0 - pow(0.9, double(k));
-        nh = (k < a.nhMax + 1) ? k - 1 : a.nhMax;
+        lambda = 1.0 - pow(0.9, double(k + (F == 1)));
+        nh = (k < a.nhMax + 1) ? k - 1 : a.nhMax - (F == 5);
+        const int us = (F == 4) ? a.nhMax : nh;  // U's row stride
         for (m = 0; m <= nh; m++) {
             double s = 0.0;
             for (int r = 0; r < M; r++) s += Dh(0)[r] * Dh(m)[r];
@@ -517,24 +540,35 @@ static int anderson_mix(orc_anderson& a, orc_diblock* dbc, int maxIter, double e
             for (m = 1; m <= nh; m++) {
                 a.V[m - 1] = DD(0, 0) - DD(0, m);
                 for (n = 1; n <= m; n++)
-                    a.U[(m - 1) * nh + n - 1] = a.U[(n - 1) * nh + m - 1] =
+                    a.U[(m - 1) * us + n - 1] = a.U[(n - 1) * us + m - 1] =
                         DD(0, 0) - DD(0, m) - DD(0, n) + DD(n, m);
             }
             lu_solve(nh, a.U.data(), a.V.data(), a.C.data());
             for (int r = 0; r < M; r++) w[r + M] += lambda * Dh(0)[r];
+            const double lh = (F == 6) ? 0.0 : lambda;
             for (n = 1; n <= nh; n++) {
                 const double* dn = Dh(n); const double* wn = wh(n);
                 const double* d0 = Dh(0); const double* w0 = wh(0);
                 for (int r = 0; r < M; r++)
-                    w[r + M] += a.C[n - 1] * ((wn[r] + lambda * dn[r]) - (w0[r] + lambda * d0[r]));
+                    w[r + M] += a.C[n - 1] * ((wn[r] + lh * dn[r]) - (w0[r] + lambda * d0[r]));
             }
         }
-        n = 1 + (k - 1) % (a.nhMax);
+        n = 1 + (k - 1 + (F == 2)) % (a.nhMax);
+        if (a.trace && a.trace_n < a.trace_cap) {
+            double* t = a.trace + (size_t)a.trace_n++ * am_trace_len(a);
+            t[0] = k; t[1] = nh; t[2] = lambda; t[3] = err; t[4] = n;
+            for (m = 0; m < a.nhMax; m++) t[5 + m] = (k >= 2 && m < nh) ? a.C[m] : 0.0;
+            t += 5 + a.nhMax;
+            std::memcpy(t, Dh(0), sizeof(double) * M);
+            std::memcpy(t + M, wh(0), sizeof(double) * M);
+            std::memcpy(t + 2 * (size_t)M, w + M, sizeof(double) * M);
+        }
         std::memcpy(Dh(n), Dh(0), sizeof(double) * M);
         std::memcpy(wh(n), wh(0), sizeof(double) * M);
         DD(n, n) = DD(0, 0);
         for (m = 1; m < n; m++) DD(m, n) = DD(0, m);
-        for (m = n + 1; m <= nh; m++) DD(n, m) = DD(0, m);
+        if (F != 3)
+            for (m = n + 1; m <= nh; m++) DD(n, m) = DD(0, m);
     }
     a.last_err = err;
     return k;
@@ -560,6 +594,10 @@ struct orc_strfunc {
     std::vector<int> wt, P;
     fft_plan1d PL[3];
     std::vector<cplx> z;
+    // planted misreading of src/strFunc.cc (tests only; 0 = none): 1 weight 2 on the
+    // k2 = mz/2 plane; 2 n_same not reset between bins; 3 the -0.5/chi_b term dropped;
+    // 4 bins split on every change of K (dK ignored)
+    int fault = 0;
 };
 
 static void sk_init(orc_strfunc& s, const int* m, const double* L, double CV, double chi_b, double dK) {
@@ -614,13 +652,18 @@ static void sk_save(orc_strfunc& s, const std::string& fileName, int dp) {
     std::ofstream out(fileName);
     out.precision(dp);
     out.setf(std::ios::fixed, std::ios::floatfield);
+    const int F = s.fault;
+    const int mk2 = s.m[2] / 2 + 1;
     for (int k = 0; k < s.Mk; k++) {
-        S_sum += s.wt[s.P[k]] * ((s.coeff / s.nsamples) * s.S[s.P[k]] - 0.5 / s.chi_b);
-        n_same += s.wt[s.P[k]];
-        if ((k == s.Mk - 1) || (fabs(s.K[s.P[k + 1]] - s.K[s.P[k]]) > s.dK)) {
+        int wt = s.wt[s.P[k]];
+        if (F == 1 && s.m[2] % 2 == 0 && s.P[k] % mk2 == mk2 - 1) wt = 2;
+        S_sum += wt * ((s.coeff / s.nsamples) * s.S[s.P[k]] - (F == 3 ? 0.0 : 0.5 / s.chi_b));
+        n_same += wt;
+        const double dK = (F == 4) ? 0.0 : s.dK;
+        if ((k == s.Mk - 1) || (fabs(s.K[s.P[k + 1]] - s.K[s.P[k]]) > dK)) {
             out << s.K[s.P[k]] << "\t" << S_sum / n_same << std::endl;
             S_sum = 0.0;
-            n_same = 0;
+            if (F != 2) n_same = 0;
         }
     }
 }
@@ -787,6 +830,14 @@ int orc_am_mix(void* a, void* dbc, int maxIter, double tol, double* w) {
     return anderson_mix(*static_cast<orc_anderson*>(a), static_cast<orc_diblock*>(dbc), maxIter, tol, w);
 }
 double orc_am_last_err(void* a) { return static_cast<orc_anderson*>(a)->last_err; }
+// Test hooks: record every iteration of the following mix calls into trace (cap records
+// of am_trace_len doubles; NULL stops recording), plant a fault (0 = none).
+void orc_am_set_trace(void* ap, double* trace, int cap) {
+    orc_anderson* a = static_cast<orc_anderson*>(ap);
+    a->trace = trace; a->trace_cap = trace ? cap : 0; a->trace_n = 0;
+}
+int orc_am_trace_count(void* a) { return static_cast<orc_anderson*>(a)->trace_n; }
+void orc_am_set_fault(void* a, int fault) { static_cast<orc_anderson*>(a)->fault = fault; }
 
 void orc_lu_solve(int n, const double* A, const double* b, double* x) { lu_solve(n, A, b, x); }
 
@@ -823,6 +874,7 @@ void orc_sk_get(void* sp, double* S, double* K, int* wt, int* P, int* ns) {
     if (ns) *ns = s->nsamples;
 }
 void orc_sk_save(void* s, const char* fileName, int dp) { sk_save(*static_cast<orc_strfunc*>(s), fileName, dp); }
+void orc_sk_set_fault(void* s, int fault) { static_cast<orc_strfunc*>(s)->fault = fault; }
 
 // ---- parameters --------------------------------------------------------------------
 double orc_z_inf(const double* L, const int* m, int N, double Nbar, int tmax) {
