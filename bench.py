@@ -108,25 +108,37 @@ def cpu_baseline(w_state, params, m, Lb, N, NA, seconds_hint=30.0):
     }
 
 
+BATCH_MESHES = [32] * 24 + [48] * 24 + [64] * 16
+
+
+def batched_member(r, n):
+    """Member r (mesh n^3) of the batched workload: N=90, NA uniform in [30,60], XbN
+    uniform in [12,20], box edge uniform in [3.5,5.0], fields N(0, 0.1^2), all from numpy
+    seed 1000+r; noise from mt19937_64 seeded 1000+r with sigma of src/lfts_params.cc:117
+    (C=50, Ndt=0.5)."""
+    g = np.random.default_rng(1000 + r)
+    NA = int(g.integers(30, 61))
+    XbN = float(g.uniform(12.0, 20.0))
+    Lb = float(g.uniform(3.5, 5.0))
+    M = n ** 3
+    w = np.concatenate([g.normal(0.0, 0.1, M), g.normal(0.0, 0.1, M)])
+    sigma = math.sqrt(2.0 * M * 0.5 / (50.0 * Lb ** 3))
+    return dict(n=n, N=90, NA=NA, XbN=XbN, L=Lb, M=M, w=w, sigma=sigma, seed=1000 + r, dt=0.5)
+
+
+def batched_members(quick=False):
+    meshes = [32] * 3 + [48] * 3 + [64] * 2 if quick else BATCH_MESHES
+    return [batched_member(r, n) for r, n in enumerate(meshes)]
+
+
 def batched_leg(steps, warmup, L_mod, torch, quick=False):
     """Replica set of the batched workload: 64 members (24 x 32^3, 24 x 48^3, 16 x 64^3),
     N=90, NA uniform in [30,60], XbN uniform in [12,20], box edge uniform in [3.5,5.0],
     fields N(0, 0.1^2), member r seeded 1000+r (numpy) for its fields and parameters and
     1000+r (mt19937_64) for its noise. Times `steps` whole Langevin steps of the set, then
     the same steps of every member run alone, one after the other, from the same states."""
-    meshes = [32] * 24 + [48] * 24 + [64] * 16
-    if quick:
-        meshes = [32] * 3 + [48] * 3 + [64] * 2
-    members = []
-    for r, n in enumerate(meshes):
-        g = np.random.default_rng(1000 + r)
-        NA = int(g.integers(30, 61))
-        XbN = float(g.uniform(12.0, 20.0))
-        Lb = float(g.uniform(3.5, 5.0))
-        M = n ** 3
-        w = np.concatenate([g.normal(0.0, 0.1, M), g.normal(0.0, 0.1, M)])
-        sigma = math.sqrt(2.0 * M * 0.5 / (50.0 * Lb ** 3))
-        members.append(dict(n=n, NA=NA, XbN=XbN, L=Lb, M=M, w=w, sigma=sigma, seed=1000 + r))
+    members = batched_members(quick)
+    meshes = [mb["n"] for mb in members]
     ctxs = []
     for mb in members:
         c = L_mod.Context(90, mb["NA"], (mb["n"],) * 3, (mb["L"],) * 3)

@@ -34,7 +34,7 @@ typedef struct lfts_desc {
     int NA;       /* A-block length; NB = N - NA */
     int m[3];     /* mesh points mx, my, mz */
     double L[3];  /* box edges in units of R0 */
-    int nhMax;    /* Anderson history depth; the reference uses 10 (src/lfts_simulation.cc:88) */
+    int nhMax;    /* Anderson history depth; the reference uses 10 (src/lfts_simulation.cc:34) */
 } lfts_desc;
 
 typedef struct lfts_ctx lfts_ctx;
@@ -225,9 +225,9 @@ int lfts_langevin_full_step_nk_dev(lfts_ctx* ctx, const double* eta_new_dev, dou
                                    const lfts_nk_opts* opts, double* lnQ, lfts_nk_info* info);
 
 /* Host random stream of the reference: std::mt19937_64 seeded by the caller (the
- * reference seeds 123456789, src/lfts_simulation.cc:77) feeding one persistent
+ * reference seeds 123456789, src/lfts_simulation.cc:23) feeding one persistent
  * std::normal_distribution<double>(0, sigma) (src/langevin.cc:13,16,28); uniforms use a
- * fresh std::uniform_real_distribution<double>(0,1) per call (src/lfts_simulation.cc:157). */
+ * fresh std::uniform_real_distribution<double>(0,1) per call (src/lfts_simulation.cc:151-153). */
 typedef struct lfts_rng lfts_rng;
 int lfts_rng_create(unsigned long long seed, double sigma, lfts_rng** out);
 int lfts_rng_normals(lfts_rng* rng, double* out, long long n);

@@ -30,7 +30,7 @@ lfts_simulation::lfts_simulation(const std::string& inputFile, const run_options
     P_->outputParameters();
     M_ = P_->M();
 
-    // Random number generator, seeded as the reference (src/lfts_simulation.cc:77)
+    // Random number generator, seeded as the reference (src/lfts_simulation.cc:23)
     lfts_rng_create(123456789ULL, P_->sigma(), &rng_);
 
     w_.assign(4 * M_, 0.0);
@@ -62,7 +62,7 @@ lfts_simulation::lfts_simulation(const std::string& inputFile, const run_options
     lfts_rng_normals(rng_, eta_[0].data(), M_);
     check(lfts_langevin_init(ctx_, eta_[0].data()), "langevin init");
 
-    // Fields: from the file, or generated (src/lfts_simulation.cc:99-103)
+    // Fields: from the file, or generated (src/lfts_simulation.cc:45-49)
     if (P_->loadType() == 1) {
         std::cout << "Loading input field..." << std::endl;
         file_IO::readArray(w_.data(), inputFile, 2 * M_, 3);

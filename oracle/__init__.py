@@ -59,6 +59,7 @@ def lib():
             "orc_am_trace_count": (ctypes.c_int, [_vp]),
             "orc_am_set_fault": (None, [_vp, ctypes.c_int]),
             "orc_sk_set_fault": (None, [_vp, ctypes.c_int]),
+            "orc_set_order": (None, [ctypes.c_int]),
             "orc_lu_solve": (None, [ctypes.c_int, _dp, _dp, _dp]),
             "orc_langevin_update": (None, [_dp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
             "orc_normals": (None, [ctypes.c_ulonglong, ctypes.c_double, ctypes.c_int, _dp]),
@@ -210,6 +211,13 @@ class Anderson:
                             Dh0=t[5 + H:5 + H + M].copy(), wh0=t[5 + H + M:5 + H + 2 * M].copy(),
                             w_new=t[5 + H + 2 * M:5 + H + 3 * M].copy()))
         return out
+
+
+def set_order(order: int) -> None:
+    """Rounding-order variant of the whole oracle (0 = the reference's order): bit 0 runs
+    the 3D transforms along x, y, z; bit 1 sums the reductions from the last point. Used
+    only to measure how far two correct evaluations of a run drift apart."""
+    lib().orc_set_order(int(order))
 
 
 def lu_solve(A: np.ndarray, b: np.ndarray) -> np.ndarray:

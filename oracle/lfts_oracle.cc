@@ -131,9 +131,27 @@ static void fft_1d(const fft_plan1d& P, cplx* x, cplx* y, int sign) {
 }
 
 // 3D transform of a row-major m[0] x m[1] x m[2] complex array (last index fastest)
+// Test hook: the same arithmetic in another rounding order, to measure how far two
+// correct evaluations of one run drift apart (never used for parity or timing).
+// bit 0: the 3D transforms run along x, y, z instead of z, y, x; bit 1: the reductions
+// (Q, the Anderson residual and Gram row, the sums of getH) run from the last point to
+// the first.
+static int g_order = 0;
+
+// sum of a[0..n) in the reference's serial order, or from the last element (g_order)
+static double osum(const double* a, long n) {
+    double s = 0.0;
+    if (g_order & 2)
+        for (long r = n - 1; r >= 0; r--) s += a[r];
+    else
+        for (long r = 0; r < n; r++) s += a[r];
+    return s;
+}
+
 static void fft_3d(const fft_plan1d* P, const int* m, cplx* a, int sign) {
     const long M = (long)m[0] * m[1] * m[2];
-    for (int ax = 2; ax >= 0; ax--) {
+    for (int step = 0; step < 3; step++) {
+        const int ax = (g_order & 1) ? step : 2 - step;
         const int n = m[ax];
         if (n == 1) continue;
         long stride = 1;
@@ -251,8 +269,7 @@ static double diblock_calc_concs(orc_diblock& d, double* w) {
     // chain (src/diblock.cc:43)
     for (int i = 1; i < N; i++) step_fwd(d.st, d.q1(i), d.q1(i + 1), d.h.data(), i);
     // Q (src/diblock.cc:46-48)
-    double Q = 0.0;
-    for (int r = 0; r < M; r++) Q += d.q1(N)[r];
+    double Q = osum(d.q1(N), M);
     Q /= M;
     // phi sums (src/diblock.cc:51-53, 76-80)
     for (int r = 0; r < 2 * M; r++) phiA[r] = 0.0;
@@ -523,15 +540,16 @@ static int anderson_mix(orc_anderson& a, orc_diblock* dbc, int maxIter, double e
     for (k = 1; k < maxIter && err > errTol; k++) {
         diblock_calc_concs(*dbc, w);
         for (int r = 0; r < M; r++) { Dh(0)[r] = w[r + 3 * M] - 1.0; wh(0)[r] = w[r + M]; }
+        const bool rev = (g_order & 2) != 0;
         S1 = 0.0;
-        for (int r = 0; r < M; r++) S1 += pow(Dh(0)[r], 2.0);
+        for (int t = 0; t < M; t++) { const int r = rev ? M - 1 - t : t; S1 += pow(Dh(0)[r], 2.0); }
         err = pow(S1 / M, 0.5);
         lambda = 1.0 - pow(0.9, double(k + (F == 1)));
         nh = (k < a.nhMax + 1) ? k - 1 : a.nhMax - (F == 5);
         const int us = (F == 4) ? a.nhMax : nh;  // U's row stride
         for (m = 0; m <= nh; m++) {
             double s = 0.0;
-            for (int r = 0; r < M; r++) s += Dh(0)[r] * Dh(m)[r];
+            for (int t = 0; t < M; t++) { const int r = rev ? M - 1 - t : t; s += Dh(0)[r] * Dh(m)[r]; }
             DD(0, m) = s;
         }
         if (k < 2) {
@@ -996,11 +1014,13 @@ double orc_sim_getH(void* sp) {
     orc_sim* s = static_cast<orc_sim*>(sp);
     const int M = s->p.M;
     double lnQ = diblock_calc_concs(*s->dbc, s->w.data());
-    double wp_sum = 0.0, wm2_sum = 0.0;
-    for (int r = 0; r < M; r++) wp_sum += s->w[r + M];
-    for (int r = 0; r < M; r++) wm2_sum += pow(s->w[r], 2.0);
+    std::vector<double> sq(M);
+    for (int r = 0; r < M; r++) sq[r] = pow(s->w[r], 2.0);
+    const double wp_sum = osum(s->w.data() + M, M), wm2_sum = osum(sq.data(), M);
     return -lnQ + (wm2_sum / s->p.chi_b - wp_sum) / M;
 }
+
+void orc_set_order(int order) { g_order = order; }
 
 void orc_sim_sk_get(void* sp, double* S, int* ns) {
     orc_sim* s = static_cast<orc_sim*>(sp);
