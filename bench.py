@@ -131,7 +131,7 @@ def batched_members(quick=False):
     return [batched_member(r, n) for r, n in enumerate(meshes)]
 
 
-def batched_leg(steps, warmup, L_mod, torch, quick=False):
+def batched_leg(steps, warmup, L_mod, torch, quick=False, sequential=True):
     """Replica set of the batched workload: 64 members (24 x 32^3, 24 x 48^3, 16 x 64^3),
     N=90, NA uniform in [30,60], XbN uniform in [12,20], box edge uniform in [3.5,5.0],
     fields N(0, 0.1^2), member r seeded 1000+r (numpy) for its fields and parameters and
@@ -176,15 +176,17 @@ def batched_leg(steps, warmup, L_mod, torch, quick=False):
     # bytes moved per the contour-step accounting (48 M per member step)
     cs_bytes = sum(48.0 * mb["M"] * (90 - 1) * (int(k) - 1 + 1) for ks in kb for mb, k in zip(members, ks))
     # the same steps, member by member
-    for c, (w, _) in zip(ctxs, snap):
-        c.set_fields(w)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for r, c in enumerate(ctxs):
-        for s in range(warmup, nsteps):
-            c.full_step_dev(etas[s][r].data_ptr(), XbN[r], 0.5)
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t0
+    t_seq = float("nan")
+    if sequential:
+        for c, (w, _) in zip(ctxs, snap):
+            c.set_fields(w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r, c in enumerate(ctxs):
+            for s in range(warmup, nsteps):
+                c.full_step_dev(etas[s][r].data_ptr(), XbN[r], 0.5)
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t0
     n_members = len(members)
     return {
         "members": n_members,
@@ -314,6 +316,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=3, help="timed steps of the replica-set leg (0 skips it)")
     ap.add_argument("--batched-quick", action="store_true", help="8 members instead of 64 (smoke runs)")
     ap.add_argument("--only-batched", action="store_true", help="run only the replica-set leg (profiling)")
+    ap.add_argument("--batched-no-sequential", action="store_true",
+                    help="skip the member-by-member comparison of the replica-set leg (profiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the Newton-Krylov / derivative leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -326,7 +330,8 @@ def main():
     import lfts_cpu_amd as L
 
     if args.only_batched:
-        print(json.dumps({"batched": batched_leg(args.batched_steps, 1, L, torch, quick=args.batched_quick)}))
+        print(json.dumps({"batched": batched_leg(args.batched_steps, 1, L, torch, quick=args.batched_quick,
+                                                 sequential=not args.batched_no_sequential)}))
         return
 
     N, NA, XbN, C, dt = 90, 45, 17.0, 50.0, 0.5

@@ -26,7 +26,9 @@ enum {
     LFTS_EINVAL = 1,   /* bad argument or unsupported mesh size */
     LFTS_EHIP = 2,     /* HIP runtime error (device missing, out of memory, launch failure) */
     LFTS_ESTATE = 3,   /* call out of order (e.g. S(k) read before any sample) */
-    LFTS_ENUM = 4      /* non-finite value detected (NaN/Inf guard) */
+    LFTS_ENUM = 4,     /* non-finite value detected (NaN/Inf guard) */
+    LFTS_ESING = 5     /* singular Anderson system: the history matrix has a zero pivot (the
+                          reference's GSL LU_solve reports it and GSL's handler aborts) */
 };
 
 typedef struct lfts_desc {
@@ -66,6 +68,12 @@ int lfts_calc_concs(lfts_ctx* ctx, double* lnQ);
  * solve for w+. *k_ret receives mix's return value (executed iterations + 1); *err_ret
  * (may be NULL) the last residual. The small LU solve runs on the host. */
 int lfts_anderson_mix(lfts_ctx* ctx, int maxIter, double errTol, int* k_ret, double* err_ret);
+
+/* anderson::LUdecomp(A, Y, X, n) (src/anderson.cc:106-127): the host-side solve of
+ * A x = y for the n x n Anderson system (n <= 15), LU with partial pivoting as GSL's
+ * LU_decomp/LU_solve. A is row-major and is not modified. Returns LFTS_ESING, leaving x
+ * untouched, when U has a zero pivot. No context needed. */
+int lfts_lu_solve(int n, const double* A, const double* y, double* x);
 
 /* langevin(RNG, sigma, M) constructor's first draw (src/langevin.cc:7-19): sets the
  * "previous" noise eta_prev (M host doubles). */

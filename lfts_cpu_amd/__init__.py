@@ -17,7 +17,7 @@ DRIVER_PATH = os.path.join(PKG, "bin", "lfts-gpu")
 VTK_PATH = os.path.join(PKG, "bin", "lfts-vtk")
 BATCH_DRIVER_PATH = os.path.join(PKG, "bin", "lfts-gpu-batch")
 
-LFTS_OK, LFTS_EINVAL, LFTS_EHIP, LFTS_ESTATE, LFTS_ENUM = range(5)
+LFTS_OK, LFTS_EINVAL, LFTS_EHIP, LFTS_ESTATE, LFTS_ENUM, LFTS_ESING = range(6)
 
 
 class LftsError(RuntimeError):
@@ -81,6 +81,7 @@ SIGNATURES = {
     "lfts_sk_get": (ctypes.c_int, [_vp, _dp, _ip]),
     "lfts_sk_reset": (ctypes.c_int, [_vp]),
     "lfts_sk_bins": (ctypes.c_int, [_ip, _dp, _dp, _ip, _ip]),
+    "lfts_lu_solve": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp]),
     "lfts_sk_save": (ctypes.c_int, [ctypes.c_char_p, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "lfts_hamiltonian": (ctypes.c_int, [_vp, ctypes.c_double, _dp]),
@@ -176,12 +177,19 @@ class Context:
         self.h = h
 
     def close(self):
+        """Release the context; refused (LftsError, LFTS_ESTATE) while a replica set in use
+        holds it."""
         if getattr(self, "h", None):
-            lib().lfts_destroy(self.h)
+            rc = lib().lfts_destroy(self.h)
+            if rc != LFTS_OK:
+                raise LftsError(rc, lib().lfts_last_error(self.h).decode())
             self.h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def _chk(self, rc: int):
         if rc != LFTS_OK:
@@ -460,6 +468,18 @@ class Rng:
         out = np.zeros(int(n))
         lib().lfts_rng_uniforms(self.h, _d(out), int(n))
         return out
+
+
+def lu_solve(A: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """anderson::LUdecomp (src/anderson.cc:106-127) on the host: solves A x = y; raises
+    LftsError (LFTS_ESING) when U has a zero pivot."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.zeros(y.size)
+    rc = lib().lfts_lu_solve(int(y.size), _d(A), _d(y), _d(x))
+    if rc != LFTS_OK:
+        raise LftsError(rc, "singular Anderson system" if rc == LFTS_ESING else "bad argument")
+    return x
 
 
 def sk_bins(m, L):

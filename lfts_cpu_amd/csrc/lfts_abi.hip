@@ -173,7 +173,9 @@ int line_contour_step(lfts_ctx* c, const double* in, double* out, const double* 
 }
 
 // LU with partial pivoting and solve (the GSL LU_decomp/LU_solve of src/anderson.cc:118-126).
-void lu_solve(int n, const double* A_in, const double* b, double* x) {
+// Returns false, leaving x untouched, when U has a zero pivot: GSL's LU_solve then reports
+// the matrix as singular (and its default error handler aborts the reference run).
+bool lu_solve(int n, const double* A_in, const double* b, double* x) {
     std::vector<double> A(A_in, A_in + (size_t)n * n);
     std::vector<int> p(n);
     std::iota(p.begin(), p.end(), 0);
@@ -196,6 +198,8 @@ void lu_solve(int n, const double* A_in, const double* b, double* x) {
             for (int k = j + 1; k < n; k++) A[(size_t)i * n + k] -= l * A[(size_t)j * n + k];
         }
     }
+    for (int j = 0; j < n; j++)
+        if (A[(size_t)j * n + j] == 0.0) return false;
     std::vector<double> y(n);
     for (int i = 0; i < n; i++) {
         double s = b[p[i]];
@@ -207,6 +211,7 @@ void lu_solve(int n, const double* A_in, const double* b, double* x) {
         for (int k = i + 1; k < n; k++) s -= A[(size_t)i * n + k] * x[k];
         x[i] = s / A[(size_t)i * n + i];
     }
+    return true;
 }
 
 // ---- replica sets ----------------------------------------------------------------------
@@ -518,7 +523,11 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
                         c->U[(m - 1) * nh + nn - 1] = c->U[(nn - 1) * nh + m - 1] =
                             DD(0, 0) - DD(0, m) - DD(0, nn) + DD(nn, m);
                 }
-                lu_solve(nh, c->U.data(), c->V.data(), c->C.data());
+                if (!lu_solve(nh, c->U.data(), c->V.data(), c->C.data())) {
+                    (void)sync_batch(b);
+                    return bfail(b, LFTS_ESING, "singular Anderson system: the history matrix U has a zero pivot "
+                                                "(src/anderson.cc:124-125; GSL reports it singular and aborts)");
+                }
                 for (int nn = 0; nn < nh; nn++) co.C[nn] = c->C[nn];
             }
             co.slot = 1 + (k[id] - 1) % c->nhMax;
@@ -667,6 +676,9 @@ int device_anderson_mix(lfts_ctx* c, int maxIter, double errTol, int* k_ret, dou
     c->n_cs = cs0 + (long long)(k - 1) * (c->N - 1);
     if (k_ret) *k_ret = k;
     if (err_ret) *err_ret = err;
+    if (h->singular)
+        return fail(c, LFTS_ESING, "singular Anderson system: the history matrix U has a zero pivot "
+                                   "(src/anderson.cc:124-125; GSL reports it singular and aborts)");
     return std::isfinite(err) ? LFTS_OK : fail(c, LFTS_ENUM, "Anderson residual is not finite");
 }
 
@@ -809,6 +821,7 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
 
 int lfts_destroy(lfts_ctx* c) {
     if (!c) return LFTS_EINVAL;
+    if (c->n_sets > 0) return fail(c, LFTS_ESTATE, "destroy: a replica set in use holds this context; destroy it first");
     if (c->self) lfts_batch_destroy(c->self);
     release_pair_storage(c);
     if (c->am_exec) (void)hipGraphExecDestroy(c->am_exec);
@@ -962,6 +975,12 @@ int lfts_sk_reset(lfts_ctx* c) {
     return LFTS_OK;
 }
 
+// src/anderson.cc:106-127 (the host LU of the Anderson system)
+int lfts_lu_solve(int n, const double* A, const double* y, double* x) {
+    if (n < 1 || n > lfts::MAX_HIST || !A || !y || !x) return LFTS_EINVAL;
+    return lu_solve(n, A, y, x) ? LFTS_OK : LFTS_ESING;
+}
+
 // src/strFunc.cc:27-31, 80-103
 int lfts_sk_bins(const int* m, const double* L, double* K, int* wt, int* P) {
     if (!m || !L || !K || !wt || !P) return LFTS_EINVAL;
@@ -1082,15 +1101,23 @@ int lfts_batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
         for (int j = 0; j < i; j++)
             if (ctxs[j] == ctxs[i]) return LFTS_EINVAL;
         if (ctxs[i]->ck_stride) return LFTS_EINVAL;  // members keep every pair
+        if (ctxs[i]->n_sets > 0) return fail(ctxs[i], LFTS_ESTATE, "batch_create: the context belongs to another set in use");
         // work queued on the contexts' own streams finishes first
         if (hipStreamSynchronize(ctxs[i]->self->st) != hipSuccess) return LFTS_EHIP;
     }
-    return batch_create(ctxs, n, out);
+    const int rc = batch_create(ctxs, n, out);
+    if (rc == LFTS_OK) {
+        (*out)->external = true;
+        for (int i = 0; i < n; i++) ctxs[i]->n_sets++;
+    }
+    return rc;
 }
 
 int lfts_batch_destroy(lfts_batch* b) {
     if (!b) return LFTS_EINVAL;
     if (b->st) (void)hipStreamSynchronize(b->st);
+    if (b->external)
+        for (lfts_ctx* c : b->ctx) c->n_sets--;
     for (auto& q : b->ev_pending)
         for (hipEvent_t e : {q.pe0, q.ch0, q.ch1, q.pe1}) b->ev_pool.push_back(e);
     for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);

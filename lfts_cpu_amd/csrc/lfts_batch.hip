@@ -484,17 +484,24 @@ __global__ __launch_bounds__(64) void k_am_solve(AmState* __restrict__ s, const 
                     for (int kk = j + 1; kk < nh; kk++) U[i * nh + kk] -= l * U[j * nh + kk];
                 }
             }
-            for (int i = 0; i < nh; i++) {
-                double t = V[p[i]];
-                for (int kk = 0; kk < i; kk++) t -= U[i * nh + kk] * y[kk];
-                y[i] = t;
+            bool sing = false;
+            for (int i = 0; i < nh; i++) sing |= U[i * nh + i] == 0.0;
+            if (sing) {
+                // the host LU's zero-pivot rule: no solve, the loop stops, the call fails
+                s->singular = 1;
+            } else {
+                for (int i = 0; i < nh; i++) {
+                    double t = V[p[i]];
+                    for (int kk = 0; kk < i; kk++) t -= U[i * nh + kk] * y[kk];
+                    y[i] = t;
+                }
+                for (int i = nh - 1; i >= 0; i--) {
+                    double t = y[i];
+                    for (int kk = i + 1; kk < nh; kk++) t -= U[i * nh + kk] * C[kk];
+                    C[i] = t / U[i * nh + i];
+                }
+                nh_upd = nh;
             }
-            for (int i = nh - 1; i >= 0; i--) {
-                double t = y[i];
-                for (int kk = i + 1; kk < nh; kk++) t -= U[i * nh + kk] * C[kk];
-                C[i] = t / U[i * nh + i];
-            }
-            nh_upd = nh;
         }
         const int slot = 1 + (k - 1) % nhMax;
         DD[slot * H + slot] = DD[0];
@@ -508,7 +515,7 @@ __global__ __launch_bounds__(64) void k_am_solve(AmState* __restrict__ s, const 
         c.nh_dots = (k + 1 < nhMax + 1) ? k : nhMax;
         s->err = err;
         s->k = k + 1;
-        s->stop_next = !(k + 1 < s->maxIter && err > s->tol);
+        s->stop_next = s->singular || !(k + 1 < s->maxIter && err > s->tol);
     }
 writeback:
     __syncthreads();

@@ -455,6 +455,13 @@ extern "C" {
 
 int lfts_set_pair_storage(lfts_ctx* c, int stride) {
     if (!c || stride < 0) return fail(c, LFTS_EINVAL, "set_pair_storage: bad argument");
+    // a replica set in use holds copies of the pair addresses, which this call replaces
+    if (c->n_sets > 0) return fail(c, LFTS_ESTATE, "set_pair_storage: destroy the replica set holding this context first");
+    // so does the captured device Anderson iteration (descriptor passed by value)
+    if (c->am_exec) (void)hipGraphExecDestroy(c->am_exec);
+    if (c->am_graph) (void)hipGraphDestroy(c->am_graph);
+    c->am_exec = nullptr;
+    c->am_graph = nullptr;
     return set_pair_storage(c, stride);
 }
 

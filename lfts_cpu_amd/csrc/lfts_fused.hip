@@ -102,6 +102,7 @@ __device__ __forceinline__ double2 ld2_nt(const double* p) {
     return make_double2(w.x, w.y);
 }
 __device__ __forceinline__ void st_nt(double* p, double v) { __builtin_nontemporal_store(v, (gd*)(p)); }
+__device__ __forceinline__ void st_g(double* p, double v) { *(gd*)(p) = v; }
 __device__ __forceinline__ void st2_nt(double* p, double2 v) {
     const d2v w = {v.x, v.y};
     __builtin_nontemporal_store(w, (gd2v*)(p));
@@ -624,7 +625,12 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 // FIN = 0 only for a chain's first pass (no step to finish): fixed per kernel, so the
 // code is straight and the compiler's wait counts stay exact (a run-time branch around
 // the finish would make it wait for every store before the next convolution).
-template <int E, int R1, int R2, int MODE, int TWL, int FPW, int FUSE, int FIN>
+// TNT: the partially convolved pair T goes out with non-temporal stores (one replica:
+// a cached line left dirty in L2 is written back when the kernel ends, which lengthens
+// the gap to the dependent next pass) or with ordinary stores (replica set: its T of all
+// members, 122 MB for the 64-member workload, is read back by the next launch, and
+// cached stores made the 64^3 and 48^3 group passes 9-13 % faster).
+template <int E, int R1, int R2, int MODE, int TWL, int FPW, int FUSE, int FIN, bool TNT = true>
 __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const double2* __restrict__ tw) {
     static_assert(R1 * R2 == E, "unsupported factorisation");
     constexpr int H = E / 2;           // complex lines per plane
@@ -786,8 +792,13 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
 #pragma unroll
     for (int k = 0; k < R1; k++) {
         const long p = base + (long)r * Sa + jb + R2 * k;
-        st_nt(Tout + p, w[k].x);
-        st_nt(Tout + p + (long)H * Sa, w[k].y);
+        if constexpr (TNT) {
+            st_nt(Tout + p, w[k].x);
+            st_nt(Tout + p + (long)H * Sa, w[k].y);
+        } else {
+            st_g(Tout + p, w[k].x);
+            st_g(Tout + p + (long)H * Sa, w[k].y);
+        }
     }
 }
 
@@ -800,7 +811,7 @@ __global__ __launch_bounds__((E / 2) * R2 * FPW) void k_slab(const Rep* __restri
     const Rep& R = reps[slot];  // fields read in place (a private copy would live in scratch)
     if (i > R.N - 1) return;    // this replica's chain is complete
     const SlabIO io = plain_io(R, i);
-    slab_pass<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, FIN>(io, plane, tw);
+    slab_pass<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, FIN, false>(io, plane, tw);
 }
 
 // One replica, its pass addresses in the kernel arguments (no dependent descriptor load

@@ -238,6 +238,7 @@ struct AmState {
     int stop;          // loop condition failed before this iteration
     int stop_next;     // loop condition for the next iteration
     int maxIter;
+    int singular;      // the LU met a zero pivot (the loop stops; the call reports it)
     double err, tol;   // residual of the last iteration, tolerance
     double DD[(MAX_HIST + 1) * (MAX_HIST + 1)];
 };

@@ -56,6 +56,7 @@ struct lfts_ctx {
     double* hred = nullptr;   // pinned host copy of the results
     // checkpointed pair storage (ck_stride > 0): pairs kept only at segment starts
     int ck_stride = 0;
+    int n_sets = 0;  // replica sets in use (lfts_batch_create) that hold this context's addresses
     std::vector<int> ck_b;      // segment starts b_0 = 1 < ... < b_{K-1}, and b_K = N + 1
     int ck_smax = 0;            // longest segment (pairs)
     double* ck_f = nullptr;     // forward pairs at b_1..b_{K-1}, then pair N
@@ -86,6 +87,7 @@ struct lfts_ctx {
 
 struct lfts_batch {
     std::vector<lfts_ctx*> ctx;
+    bool external = false;  // made by lfts_batch_create (not a context's own set of one)
     hipStream_t st = nullptr;
     std::vector<lfts::Rep> reps;  // descriptors of all members (id = member index)
     lfts::Rep* d_ract = nullptr;  // descriptors of the active members, in launch order
@@ -158,7 +160,7 @@ int slab_variant(lfts_batch* b, int edge, int cnt, const lfts::Rep* d_reps, cons
 int read_scal(lfts_batch* b);
 int sync_batch(lfts_batch* b);
 std::vector<int> all_ids(const lfts_batch* b);
-void lu_solve(int n, const double* A_in, const double* b, double* x);
+bool lu_solve(int n, const double* A_in, const double* b, double* x);
 bool factorize(int n, std::vector<int>& r);
 
 // One propagator-type chain for run_lockstep: passes i0 .. i1-1 produce pairs

@@ -10,6 +10,7 @@
 // All inputs must share equil_its, sim_its, sample_freq and save_freq (the replicas
 // step in lockstep).
 #include <chrono>
+#include <climits>
 #include <cstdlib>
 #include <future>
 #include <iostream>
@@ -58,6 +59,18 @@ std::string dir_of(const std::string& f) {
     return p == std::string::npos ? std::string() : f.substr(0, p + 1);
 }
 
+// The directory a replica writes its files into, resolved (symbolic links, "..", the
+// working directory for a bare file name), so that two spellings of one directory compare
+// equal. Empty if it does not exist.
+std::string output_dir(const std::string& f) {
+    const std::string d = dir_of(f);
+    char* r = realpath(d.empty() ? "." : d.c_str(), nullptr);
+    if (!r) return std::string();
+    std::string out(r);
+    std::free(r);
+    return out;
+}
+
 // src/lfts_simulation.cc:139-142 and the phi file beside it
 void save_fields(Replica& r, const std::string& tag) {
     check(lfts_get_fields(r.ctx, r.w.data(), 4), r.ctx, "get fields");
@@ -81,6 +94,22 @@ int main(int argc, char* argv[]) {
         return 1;
     }
     const int n = argc - 1;
+    // Every replica writes the same file names (w_0, phi_0, w_eq_*, ...) into its input's
+    // directory, so two inputs in one directory would overwrite each other's outputs:
+    // refuse that before any work starts.
+    {
+        std::vector<std::string> dirs(n);
+        for (int i = 0; i < n; i++) {
+            const std::string in(argv[i + 1]);
+            if (!file_IO::isValidFile(in)) die("Cannot open one of the input files.");
+            dirs[i] = output_dir(in);
+            for (int j = 0; j < i; j++)
+                if (dirs[j] == dirs[i])
+                    die("inputs " + std::string(argv[j + 1]) + " and " + in + " are in the same directory (" +
+                        dirs[i] + "); each replica writes its output files into its input's directory, "
+                        "so every input needs a directory of its own");
+        }
+    }
     std::vector<Replica> rep(n);
     for (int i = 0; i < n; i++) {
         Replica& r = rep[i];

@@ -150,3 +150,39 @@ def test_rng_state_round_trip_continues_the_stream():
         assert np.array_equal(s.uniforms(5), u)
     with pytest.raises(L.LftsError):
         L.Rng(1, 1.0).load_state(b"not a state")
+
+
+# ---- the host LU of the Anderson system (anderson::LUdecomp, src/anderson.cc:106-127) ----
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 10, 15])
+def test_lu_solve_matches_lapack_and_the_oracle(n):
+    import scipy.linalg
+
+    rng = np.random.default_rng(20 + n)
+    A = rng.standard_normal((n, n)) + 0.5 * n * np.eye(n)
+    y = rng.standard_normal(n)
+    x = L.lu_solve(A, y)
+    assert np.allclose(x, scipy.linalg.solve(A, y), rtol=1e-12, atol=1e-13)
+    assert np.array_equal(x, O.lu_solve(A, y))  # same pivots, same operation order
+
+
+def test_lu_solve_pivots():
+    A = np.array([[0.0, 2.0, 1.0], [1.0, 1.0, 0.0], [3.0, 0.0, 1.0]])
+    y = np.array([1.0, 2.0, 3.0])
+    assert np.allclose(A @ L.lu_solve(A, y), y, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("A", [np.zeros((3, 3)), np.array([[1.0, 2.0], [2.0, 4.0]]),
+                               np.array([[2.0, 1.0, 1.0], [4.0, 2.0, 2.0], [1.0, 0.0, 3.0]])])
+def test_lu_solve_reports_a_singular_system(A):
+    """GSL's LU_solve reports a matrix with a zero pivot as singular (and the reference run
+    aborts); the library returns LFTS_ESING instead of dividing by zero."""
+    with pytest.raises(L.LftsError) as e:
+        L.lu_solve(A, np.ones(A.shape[0]))
+    assert e.value.code == L.LFTS_ESING
+
+
+def test_lu_solve_rejects_bad_sizes():
+    x = np.zeros(16)
+    assert L.lib().lfts_lu_solve(16, L._d(np.eye(16).reshape(-1)), L._d(np.ones(16)), L._d(x)) == L.LFTS_EINVAL
+    assert L.lib().lfts_lu_solve(0, L._d(np.eye(1).reshape(-1)), L._d(np.ones(1)), L._d(x)) == L.LFTS_EINVAL
