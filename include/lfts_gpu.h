@@ -133,6 +133,16 @@ int lfts_get_pair(lfts_ctx* ctx, int i, double* out);
 /* step::fwd(q_in, q_out, h, i) (src/step.cc:35-65) on host arrays (2*M each). */
 int lfts_step_fwd(lfts_ctx* ctx, const double* q_in, double* q_out, const double* h, int i);
 
+/* lfts_step_fwd on device arrays (q_in_dev, h_dev: 2*M doubles each; q_out_dev: 2*M,
+ * distinct from both) with no allocation: no reference counterpart; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops). The
+ * intermediate pair of the three axis passes lives in a fixed device array of the library
+ * (32 MiB on every GPU it runs on, held until the process ends), so meshes of up to
+ * 2097152 points (128^3) are served; larger ones return LFTS_EINVAL. Stream-ordered on the
+ * context's stream; no synchronisation. All calls on one GPU share that array: calls that
+ * overlap on one GPU race. Same results as lfts_step_fwd, bit for bit. */
+int lfts_step_fwd_noalloc(lfts_ctx* ctx, const double* q_in_dev, double* q_out_dev, const double* h_dev, int i);
+
 /* Counters: propagator evaluations and contour steps run on this context so far. */
 int lfts_counters(lfts_ctx* ctx, long long* n_prop_evals, long long* n_contour_steps);
 

@@ -82,6 +82,7 @@ SIGNATURES = {
     "lfts_sk_reset": (ctypes.c_int, [_vp]),
     "lfts_sk_bins": (ctypes.c_int, [_ip, _dp, _dp, _ip, _ip]),
     "lfts_lu_solve": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp]),
+    "lfts_step_fwd_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
     "lfts_sk_save": (ctypes.c_int, [ctypes.c_char_p, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "lfts_hamiltonian": (ctypes.c_int, [_vp, ctypes.c_double, _dp]),

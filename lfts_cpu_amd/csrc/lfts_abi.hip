@@ -31,13 +31,19 @@ using lfts::Rep;
 namespace lfts_abi {
 
 int fail(lfts_ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
+    if (c) {
+        c->err = msg;
+        c->err_lit = nullptr;
+    }
     return code;
 }
 int bfail(lfts_batch* b, int code, const std::string& msg) {
     if (b) {
         b->err = msg;
-        for (lfts_ctx* c : b->ctx) c->err = msg;
+        for (lfts_ctx* c : b->ctx) {
+            c->err = msg;
+            c->err_lit = nullptr;
+        }
     }
     return code;
 }
@@ -756,6 +762,10 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LFTS_EHIP;
     lfts_ctx* c = new lfts_ctx();
+    if (hipGetDevice(&c->dev) != hipSuccess) {
+        delete c;
+        return LFTS_EHIP;
+    }
     c->d = *desc;
     c->N = desc->N;
     c->NA = desc->NA;
@@ -812,6 +822,7 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     c->V.assign(c->nhMax, 0.0);
     c->C.assign(c->nhMax, 0.0);
     CREATE_CHK(batch_create(&c, 1, &c->self));
+    (void)noalloc_setup(c->dev);  // its result is kept, and reported by lfts_step_fwd_noalloc
     if (auto_stride) CREATE_CHK(set_pair_storage(c, auto_stride));
 #undef CREATE_CHK
 #undef CREATE_HIP
@@ -845,7 +856,7 @@ int lfts_destroy(lfts_ctx* c) {
     return LFTS_OK;
 }
 
-const char* lfts_last_error(const lfts_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* lfts_last_error(const lfts_ctx* c) { return c ? (c->err_lit ? c->err_lit : c->err.c_str()) : "null context"; }
 
 int lfts_get_stream(lfts_ctx* c, void** stream) {
     if (!c || !stream) return LFTS_EINVAL;

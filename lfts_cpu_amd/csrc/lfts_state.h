@@ -83,6 +83,8 @@ struct lfts_ctx {
     std::vector<double> DD, U, V, C;  // Anderson host state (src/anderson.cc:8-21)
     lfts_batch* self = nullptr;       // set of one for the single-context entry points
     std::string err;
+    const char* err_lit = nullptr;  // static text of the last failure of a call that may not allocate
+    int dev = 0;                    // the HIP device the context was created on
 };
 
 struct lfts_batch {
@@ -161,6 +163,8 @@ int read_scal(lfts_batch* b);
 int sync_batch(lfts_batch* b);
 std::vector<int> all_ids(const lfts_batch* b);
 bool lu_solve(int n, const double* A_in, const double* b, double* x);
+// lfts_noalloc.hip: the scratch array's address on device dev, taken at context creation
+int noalloc_setup(int dev);
 bool factorize(int n, std::vector<int>& r);
 
 // One propagator-type chain for run_lockstep: passes i0 .. i1-1 produce pairs
