@@ -111,6 +111,42 @@ def cpu_baseline(w_state, params, m, Lb, N, NA, seconds_hint=30.0):
 BATCH_MESHES = [32] * 24 + [48] * 24 + [64] * 16
 
 
+def cpu_baseline_torch(w_state, params, m, Lb, N, NA, sigma, steps=3):
+    """The stronger CPU baseline: the reference's Langevin step with torch's threaded CPU
+    FFT for the batched-2 r2c/c2r transforms and threaded element-wise loops (the CPU
+    restatement oracle/torch_cpu.py, checked against the oracle), timed for `steps` whole
+    Langevin steps of the same workload from the GPU's state after warm-up, on all host
+    threads torch is given (OMP_NUM_THREADS)."""
+    import torch
+
+    from oracle import torch_cpu as T
+
+    n_thr = T.set_threads()
+    M = int(np.prod(m))
+    t = T.TorchLfts(N, NA, m, Lb)
+    w = torch.from_numpy(np.array(w_state, dtype=np.float64))
+    g = np.random.default_rng(17)
+    etas = [torch.from_numpy(g.normal(0.0, sigma, M)) for _ in range(steps + 1)]
+    nam = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        t.langevin_update(w, etas[s], etas[s + 1], params["XbN"], params["dt"])
+        nam.append(t.anderson_mix(200, 1e-4, w) - 1)
+        t.calc_concs(w)
+    dt_s = time.perf_counter() - t0
+    return {
+        "value": steps / dt_s,
+        "unit": "Langevin steps/s",
+        "cores": n_thr,
+        "kind": "port",
+        "fft_library": T.fft_library(),
+        "cpu_model": T.cpu_model(),
+        "sample": f"{steps} whole Langevin steps of the same 64^3 N=90 workload from the GPU state after warm-up "
+                  f"(n_AM={nam}, {dt_s:.1f} s): the reference's arithmetic with torch's threaded CPU FFT for the "
+                  f"batched-2 r2c/c2r transforms, threaded element-wise loops, LAPACK for the Anderson LU",
+    }
+
+
 def batched_member(r, n):
     """Member r (mesh n^3) of the batched workload: N=90, NA uniform in [30,60], XbN
     uniform in [12,20], box edge uniform in [3.5,5.0], fields N(0, 0.1^2), all from numpy
@@ -393,12 +429,19 @@ def main():
     step_gbps = step_bytes / elapsed / 1e9
     cs_per_s = sum((N - 1) * (n + 1) for n in nAM) * M / elapsed
 
-    cpu = None
+    cpu = cpu_oracle = None
     if not args.no_cpu:
+        state = ctx.get_fields(4)
         try:
-            cpu = cpu_baseline(ctx.get_fields(4), params, m, Lb, N, NA)
-        except Exception as e:  # the baseline is reported, not required
-            log(f"cpu baseline failed: {e!r}")
+            cpu = cpu_baseline_torch(state, params, m, Lb, N, NA, sigma)
+            log(f"cpu baseline (torch FFT): {cpu}")
+        except Exception as e:  # the baselines are reported, not required
+            log(f"cpu baseline (torch FFT) failed: {e!r}")
+        try:
+            cpu_oracle = cpu_baseline(state, params, m, Lb, N, NA)
+            log(f"cpu baseline (oracle): {cpu_oracle}")
+        except Exception as e:
+            log(f"cpu baseline (oracle) failed: {e!r}")
 
     adjoint = None
     if not args.no_adjoint:
@@ -432,6 +475,30 @@ def main():
     torch.cuda.synchronize()
     host_rate = HOST_STEPS / (time.perf_counter() - t1)
     log(f"host-noise steps: {host_rate:.2f} steps/s")
+
+    # driver-style rate (reported, never `value`): as host/lfts_simulation.cc runs it, each
+    # step's M normals are drawn by the host stream (mt19937_64 + normal_distribution) on a
+    # second thread while the GPU runs the step before, then handed over in host memory
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(1) as ex:
+        nxt = ex.submit(rng.normals, M)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(HOST_STEPS):
+            e = nxt.result()
+            nxt = ex.submit(rng.normals, M)  # the next step's noise, drawn meanwhile
+            ctx.langevin_step(e, XbN, dt)
+            ctx.anderson_mix(200, 1e-4)
+            ctx.calc_concs()
+        torch.cuda.synchronize()
+        driver_rate = HOST_STEPS / (time.perf_counter() - t1)
+        nxt.result()
+    t1 = time.perf_counter()
+    rng.normals(M)
+    draw_ms = (time.perf_counter() - t1) * 1e3
+    log(f"driver-style steps (noise drawn on a host thread): {driver_rate:.2f} steps/s; one draw of M normals "
+        f"{draw_ms:.1f} ms")
 
     batched = None
     if args.batched_steps > 0:
@@ -480,7 +547,11 @@ def main():
         "contour_steps_M_per_s": cs_per_s,
         "propagator_eval_ms": ms_pe / max(n_pe, 1),
         "pcie_inclusive_steps_per_s": host_rate,
+        "driver_style_steps_per_s": driver_rate,
+        "host_noise_draw_ms": draw_ms,
         "cpu_baseline": cpu,
+        "cpu_baseline_oracle": cpu_oracle,
+        "speedup_vs_cpu_baseline": (steps_per_s / cpu["value"]) if cpu else None,
         "batched": batched,
         "newton_krylov": adjoint,
     }

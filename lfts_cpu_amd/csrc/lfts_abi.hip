@@ -363,7 +363,8 @@ int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep*
 
 // One propagator evaluation (src/diblock.cc:30-60) of every active replica; Q of
 // replica id lands in d_scal[id][0].
-int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots) {
+int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int part, long long* npasses_out) {
+    const bool whole = part == PART_ALL;
     const int nact = (int)b->act.size();
     if (nact == 0) return LFTS_OK;
     if (b->ctx.size() == 1 && b->ctx[0]->ck_stride) {  // checkpointed storage: its own phi pass
@@ -378,15 +379,16 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots) {
     const int fuse = (!store_all && nact == 1) ? 1 : 0;
     lfts_batch::EvQuad ev{};
     // live timing of every prof-th evaluation (its events cost host and queue time)
-    const bool rec = b->prof > 0 && (b->prof_seq++ % b->prof) == 0;
+    const bool rec = whole && b->prof > 0 && (b->prof_seq++ % b->prof) == 0;
     if (rec) {
         for (hipEvent_t* e : {&ev.pe0, &ev.ch0, &ev.ch1, &ev.pe1})
             if (int rc = get_event(b, e)) return rc;
         BHIPCHK(b, hipEventRecord(ev.pe0, b->st));
     }
-    if (!h_ready) BHIPCHK(b, lfts::launch_prepare_h(b->d_ract, nact, b->nb_max, b->st));
+    if (!h_ready && part != PART_PHI) BHIPCHK(b, lfts::launch_prepare_h(b->d_ract, nact, b->nb_max, b->st));
     if (rec) BHIPCHK(b, hipEventRecord(ev.ch0, b->st));
     long long npasses = 0;
+    if (part != PART_PHI) {
     // slab passes: i = 0 prologue, i = 1..N-1 contour steps (src/diblock.cc:43). With
     // more than one mesh size each group's chain runs on its own stream.
     const int ng = (int)b->groups.size();
@@ -427,6 +429,9 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots) {
             BHIPCHK(b, hipEventRecord(b->ev_join[g], b->gst[g]));
             BHIPCHK(b, hipStreamWaitEvent(b->st, b->ev_join[g], 0));
         }
+    }  // chains
+    if (npasses_out) *npasses_out = npasses;
+    if (part == PART_CHAIN) return LFTS_OK;
     if (rec) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
     BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_qpart, b->st));
     const int nh_one = (dots == DOTS_ONE) ? b->h_coef[b->act[0]].nh_dots : -1;
@@ -440,9 +445,93 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots) {
     }
     for (int id : b->act) {
         lfts_ctx* c = b->ctx[id];
-        c->n_pe++;
-        c->n_cs += c->N - 1;
+        if (whole) {  // a captured iteration counts its evaluations at each replay
+            c->n_pe++;
+            c->n_cs += c->N - 1;
+        }
         c->pairs_all = !lfts::fused_phi(make_rep(c), fuse);
+    }
+    return LFTS_OK;
+}
+
+// ---- the Anderson iteration of a replica set as one HIP graph ---------------------------
+
+// Graph mode: sets of two or more members, all on the slab path (the line-path chain takes
+// its flag from the member's own context), unless LFTS_GRAPH=0.
+bool graph_mode(const lfts_batch* b) {
+    if (b->ctx.size() < 2 || !b->d_flags) return false;
+    for (const lfts_ctx* c : b->ctx)
+        if (!c->fused || c->ck_stride) return false;
+    const char* ge = std::getenv("LFTS_GRAPH");
+    return !(ge && std::strcmp(ge, "0") == 0);
+}
+
+// flags: evaluation flag of member id set unless it is in `eval`, update flag unless in `upd`
+int put_flags(lfts_batch* b, const std::vector<int>& eval, const std::vector<int>& upd, bool upload) {
+    const int n = (int)b->ctx.size();
+    for (int i = 0; i < 2 * n; i++) b->h_flags[i] = 1;
+    for (int id : eval) b->h_flags[id] = 0;
+    for (int id : upd) b->h_flags[n + id] = 0;
+    if (upload) BHIPCHK(b, hipMemcpyAsync(b->d_flags, b->h_flags, sizeof(int) * 2 * n, hipMemcpyHostToDevice, b->st));
+    return LFTS_OK;
+}
+
+// One Anderson iteration after the first, every member in the launches, in three parts:
+// (0) flags and coefficients (pinned staging, read when the graph runs) and the update
+// with the Boltzmann factors; (1) the contour-step chains on the groups' streams; (2) Q,
+// phi with the Gram row, its reduction and read-back.
+int enqueue_iteration_part(lfts_batch* b, int part) {
+    const int n = (int)b->ctx.size();
+    if (part == 0) {
+        BHIPCHK(b, hipMemcpyAsync(b->d_flags, b->h_flags, sizeof(int) * 2 * n, hipMemcpyHostToDevice, b->st));
+        if (int rc = upload_coef(b)) return rc;
+        BHIPCHK(b, lfts::launch_am_update(b->d_rupd, n, b->nb_max, b->d_coef, 1, b->st));
+    } else if (part == 1) {
+        if (int rc = calc_concs_async(b, false, true, DOTS_COEF, PART_CHAIN, &b->it_npasses)) return rc;
+    } else {
+        if (int rc = calc_concs_async(b, false, true, DOTS_COEF, PART_PHI)) return rc;
+        BHIPCHK(b, lfts::launch_reduce(b->d_ract, n, b->d_part, b->it_ncomp, 0, b->d_scal, 0, b->st));
+        BHIPCHK(b, hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(double) * lfts::SSTRIDE * n, hipMemcpyDeviceToHost,
+                                  b->st));
+    }
+    return LFTS_OK;
+}
+
+// Replays the captured iteration (capturing its parts first) and waits for it. With live
+// timing, events recorded between the parts give the chain and evaluation times.
+int replay_iteration(lfts_batch* b) {
+    for (int part = 0; part < 3; part++) {
+        if (b->it_exec[part]) continue;
+        BHIPCHK(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_iteration_part(b, part);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->st, &g);
+        if (rc || e != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : bfail(b, LFTS_EHIP, "Anderson iteration graph: capture failed");
+        }
+        b->it_graph[part] = g;
+        BHIPCHK(b, hipGraphInstantiate(&b->it_exec[part], g, nullptr, nullptr, 0));
+    }
+    lfts_batch::EvQuad ev{};
+    if (b->prof)
+        for (hipEvent_t* e : {&ev.ch0, &ev.ch1, &ev.pe1})
+            if (int rc = get_event(b, e)) return rc;
+    BHIPCHK(b, hipGraphLaunch(b->it_exec[0], b->st));
+    if (b->prof) BHIPCHK(b, hipEventRecord(ev.ch0, b->st));
+    BHIPCHK(b, hipGraphLaunch(b->it_exec[1], b->st));
+    if (b->prof) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
+    BHIPCHK(b, hipGraphLaunch(b->it_exec[2], b->st));
+    if (b->prof) BHIPCHK(b, hipEventRecord(ev.pe1, b->st));
+    if (int rc = sync_batch(b)) return rc;
+    if (b->prof) {
+        float ch = 0.f, pe = 0.f;
+        BHIPCHK(b, hipEventElapsedTime(&ch, ev.ch0, ev.ch1));
+        BHIPCHK(b, hipEventElapsedTime(&pe, ev.ch0, ev.pe1));  // the update wrote h: the evaluation starts here
+        for (hipEvent_t e : {ev.ch0, ev.ch1, ev.pe1}) b->ev_pool.push_back(e);
+        b->prof_ms_chain += ch;
+        b->prof_ms_pe += pe;
+        b->prof_n_pass += b->it_npasses;
     }
     return LFTS_OK;
 }
@@ -492,21 +581,57 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
     // From the second iteration on, every member still active was in the previous
     // update, which also wrote its Boltzmann factors (with_h).
     bool h_ready = false;
+    // Graph mode: the first iteration runs directly with every member in the launches;
+    // each later one is a replay of the captured iteration. `upd` are the members whose
+    // update the next replay applies (those of the iteration before it).
+    const bool graph = graph_mode(b);
+    std::vector<int> upd;
+    if (graph) {
+        if (int rc = set_active(b, all_ids(b))) return rc;
+        if (!b->rupd_ready) {
+            // the update kernel's descriptors: every member in launch order, update flags
+            std::vector<lfts::Rep> ru(b->h_ract, b->h_ract + n);
+            for (lfts::Rep& r : ru) r.skip = b->d_flags + n + r.id;
+            BHIPCHK(b, hipStreamSynchronize(b->st));
+            BHIPCHK(b, hipMemcpy(b->d_rupd, ru.data(), sizeof(lfts::Rep) * n, hipMemcpyHostToDevice));
+            b->it_ncomp = 1;
+            for (const lfts_ctx* c : b->ctx) b->it_ncomp = std::max(b->it_ncomp, c->nhMax + 1);
+            b->rupd_ready = true;
+        }
+    }
     while (!active.empty()) {
-        if (active != cur) {
-            if (int rc = set_active(b, active)) return rc;
-            cur = active;
+        const int nact = graph ? n : (int)active.size();
+        if (graph) {
+            if (!h_ready) {
+                if (int rc = put_flags(b, active, {}, true)) return rc;
+                if (int rc = calc_concs_async(b, false, false, DOTS_COEF)) return rc;
+                BHIPCHK(b, lfts::launch_reduce(b->d_ract, n, b->d_part, b->it_ncomp, 0, b->d_scal, 0, b->st));
+                if (int rc = read_scal(b)) return rc;
+            } else {
+                if (int rc = put_flags(b, active, upd, false)) return rc;
+                if (int rc = replay_iteration(b)) return rc;
+                for (int id : active) {
+                    lfts_ctx* c = b->ctx[id];
+                    c->n_pe++;
+                    c->n_cs += c->N - 1;
+                }
+                if (b->prof) b->prof_n_pe += (long long)active.size();
+            }
+        } else {
+            if (active != cur) {
+                if (int rc = set_active(b, active)) return rc;
+                cur = active;
+            }
+            // the evaluation's phi pass also writes the Gram row partials (dots)
+            if (int rc = calc_concs_async(b, false, h_ready, byval ? DOTS_ONE : DOTS_COEF)) return rc;
+            int nh_max = 0;
+            for (int id : active) {
+                const lfts_ctx* c = b->ctx[id];
+                nh_max = std::max(nh_max, (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax);
+            }
+            BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, nh_max + 1, 0, b->d_scal, 0, b->st));
+            if (int rc = read_scal(b)) return rc;
         }
-        // the evaluation's phi pass also writes the Gram row partials (dots)
-        if (int rc = calc_concs_async(b, false, h_ready, byval ? DOTS_ONE : DOTS_COEF)) return rc;
-        int nh_max = 0;
-        for (int id : active) {
-            const lfts_ctx* c = b->ctx[id];
-            nh_max = std::max(nh_max, (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax);
-        }
-        const int nact = (int)b->act.size();
-        BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, nh_max + 1, 0, b->d_scal, 0, b->st));
-        if (int rc = read_scal(b)) return rc;
         for (int id : active) {
             lfts_ctx* c = b->ctx[id];
             const int H = c->nhMax + 1;
@@ -538,7 +663,9 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             }
             co.slot = 1 + (k[id] - 1) % c->nhMax;
         }
-        if (byval) {
+        if (graph) {
+            upd = active;  // applied by the next replay, or after the loop
+        } else if (byval) {
             BHIPCHK(b, lfts::launch_am_update_one(b->d_ract, b->nb_max, b->h_coef[active[0]], 1, b->st));
         } else {
             if (int rc = upload_coef(b)) return rc;
@@ -559,6 +686,16 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             if (k[id] < maxIter && err[id] > errTol) still.push_back(id);
         }
         active.swap(still);
+    }
+    if (graph) {
+        // the last update (of the members that left the loop with it), then every flag back
+        // to 0 so that the set's other calls run every member
+        if (!upd.empty()) {
+            if (int rc = put_flags(b, {}, upd, true)) return rc;
+            if (int rc = upload_coef(b)) return rc;
+            BHIPCHK(b, lfts::launch_am_update(b->d_rupd, n, b->nb_max, b->d_coef, 1, b->st));
+        }
+        if (int rc = put_flags(b, all_ids(b), all_ids(b), true)) return rc;
     }
     if (int rc = sync_batch(b)) return rc;
     bool nonfinite = false;
@@ -738,6 +875,14 @@ int batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
     BCREATE(hipMalloc(&b->d_scal, sizeof(double) * lfts::SSTRIDE * n));
     BCREATE(hipMalloc(&b->d_qpart, sizeof(double) * lfts::MAX_NB * n));
     BCREATE(hipHostMalloc(&b->h_scal, sizeof(double) * lfts::SSTRIDE * n, hipHostMallocDefault));
+    if (n > 1) {  // flags of the replayed Anderson iteration (all zero: every member runs)
+        BCREATE(hipMalloc(&b->d_flags, sizeof(int) * 2 * n));
+        BCREATE(hipMemset(b->d_flags, 0, sizeof(int) * 2 * n));
+        BCREATE(hipHostMalloc(&b->h_flags, sizeof(int) * 2 * n, hipHostMallocDefault));
+        std::memset(b->h_flags, 0, sizeof(int) * 2 * n);
+        BCREATE(hipMalloc(&b->d_rupd, sizeof(Rep) * n));
+        for (int i = 0; i < n; i++) b->reps[i].skip = b->d_flags + i;
+    }
 #undef BCREATE
     *out = b;
     return LFTS_OK;
@@ -1142,6 +1287,13 @@ int lfts_batch_destroy(lfts_batch* b) {
         if (b->ev_join[g]) (void)hipEventDestroy(b->ev_join[g]);
     }
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    for (int part = 0; part < 3; part++) {
+        if (b->it_exec[part]) (void)hipGraphExecDestroy(b->it_exec[part]);
+        if (b->it_graph[part]) (void)hipGraphDestroy(b->it_graph[part]);
+    }
+    (void)hipFree(b->d_flags);
+    (void)hipFree(b->d_rupd);
+    if (b->h_flags) (void)hipHostFree(b->h_flags);
     for (void* p : {(void*)b->d_ract, (void*)b->d_coef, (void*)b->d_part, (void*)b->d_scal, (void*)b->d_qpart})
         (void)hipFree(p);
     for (void* p : {(void*)b->h_ract, (void*)b->h_coef, (void*)b->h_scal})

@@ -123,6 +123,22 @@ struct lfts_batch {
     std::vector<EvQuad> ev_pending;
     double prof_ms_chain = 0.0, prof_ms_pe = 0.0;
     long long prof_n_pass = 0, prof_n_pe = 0;
+    // A set of two or more slab-path members runs each Anderson iteration after the first
+    // as one HIP graph (it_exec): coefficient and flag uploads, the update, the propagator
+    // evaluation on the groups' streams, the Gram row and its read-back. Every member stays
+    // in every launch of the graph; per-member flags (d_flags: n evaluation flags, then n
+    // update flags, each member's descriptors pointing at its own) make the kernels of a
+    // member that has left the loop return at once.
+    int* d_flags = nullptr;       // 2n ints
+    int* h_flags = nullptr;       // pinned staging
+    lfts::Rep* d_rupd = nullptr;  // all members' descriptors with the update flags (launch order)
+    // three graphs: before the chains (uploads, update), the chains, after them (Q, phi and
+    // the Gram row, read-back), so that live timing can record events between them
+    hipGraph_t it_graph[3] = {};
+    hipGraphExec_t it_exec[3] = {};
+    long long it_npasses = 0;     // contour-step launches of the chain graph
+    int it_ncomp = 0;             // Gram-row components the graph's reduction sums
+    bool rupd_ready = false;      // d_rupd uploaded
     std::string err;
 };
 
@@ -156,7 +172,12 @@ int set_active(lfts_batch* b, const std::vector<int>& ids);
 // history depths of the uploaded coefficients (DOTS_COEF) or, for one member, that of
 // the pinned host coefficients passed as a kernel argument (DOTS_ONE).
 enum { DOTS_NONE = 0, DOTS_COEF = 1, DOTS_ONE = 2 };
-int calc_concs_async(lfts_batch* b, bool store_all = false, bool h_ready = false, int dots = DOTS_NONE);
+// part: the whole evaluation (PART_ALL), or for a captured Anderson iteration only its
+// chains (PART_CHAIN, *npasses_out receives the contour-step launches) or only Q and phi
+// (PART_PHI); the parts record no timing events and count no evaluations
+enum { PART_ALL = 0, PART_CHAIN = 1, PART_PHI = 2 };
+int calc_concs_async(lfts_batch* b, bool store_all = false, bool h_ready = false, int dots = DOTS_NONE,
+                     int part = PART_ALL, long long* npasses_out = nullptr);
 int slab_variant(lfts_batch* b, int edge, int cnt, const lfts::Rep* d_reps, const lfts::Rep* h_reps,
                  const double2* tw, hipStream_t s, int* variant);
 int read_scal(lfts_batch* b);

@@ -105,3 +105,75 @@ def test_slab_48_against_oracle(gpu):
     for i in (2, N // 2, N):
         p, po = ctx.pair(i), d.pair(i)
         assert np.abs(p - po).max() / np.abs(po).max() < 1e-10, i
+
+
+# ---- the Anderson iteration replayed as one HIP graph ---------------------------------------
+
+SLAB_MEMBERS = [
+    # slab-path meshes only (graph mode): three sizes, mixed N and NA
+    ((32, 32, 32), (3.6, 3.6, 3.6), 16, 7, 11),
+    ((48, 48, 48), (4.1, 4.1, 4.1), 12, 9, 12),
+    ((16, 16, 16), (2.9, 2.9, 2.9), 20, 8, 13),
+    ((32, 32, 32), (3.9, 3.9, 3.9), 14, 3, 14),
+]
+
+
+def _steps(members, monkeypatch, graph, nsteps=2):
+    import torch
+
+    monkeypatch.setenv("LFTS_GRAPH", "1" if graph else "0")
+    ctxs, _ = _make(members)
+    for c in ctxs:
+        c.langevin_init(np.random.default_rng(98).normal(0, 1.0, c.M))
+    etas = [[torch.from_numpy(np.random.default_rng(50 * s + i).normal(0, 1.0, c.M)).cuda() for i, c in
+             enumerate(ctxs)] for s in range(nsteps)]
+    torch.cuda.synchronize()
+    XbN = [12.0 + i for i in range(len(ctxs))]
+    b = L.Batch(ctxs)
+    k0, _ = b.anderson_mix(200, 1e-4)
+    b.profile(True)
+    out = [b.full_step_dev([e.data_ptr() for e in etas[s]], XbN, 0.5) for s in range(nsteps)]
+    prof = b.profile_read()
+    fields = [c.get_fields(4) for c in ctxs]
+    counters = [c.counters() for c in ctxs]
+    b.close()
+    for c in ctxs:
+        c.close()
+    return k0, out, fields, counters, prof
+
+
+def test_graph_iterations_equal_direct_launches(gpu, monkeypatch):
+    """Every Anderson iteration after the first as a replay of one captured graph (all
+    members in its launches, those out of the loop skipped by flags) gives the bits of
+    the direct launches over the active members, member by member."""
+    g = _steps(SLAB_MEMBERS, monkeypatch, True)
+    d = _steps(SLAB_MEMBERS, monkeypatch, False)
+    assert np.array_equal(g[0], d[0])
+    assert len(set(g[0].tolist())) > 1  # members leave the loop at different iterations
+    for (lg, kg), (ld, kd) in zip(g[1], d[1]):
+        assert np.array_equal(lg, ld) and np.array_equal(kg, kd)
+        assert len(set(kg.tolist())) > 1
+    for fg, fd in zip(g[2], d[2]):
+        assert np.array_equal(fg, fd)
+    assert g[3] == d[3]  # the same evaluation and contour-step counts per member
+    ms_chain, n_pass, ms_pe, n_pe = g[4]
+    assert ms_chain > 0 and ms_pe >= ms_chain and n_pass > 0 and n_pe == d[4][3]
+
+
+def test_graph_set_equals_members_alone(gpu, monkeypatch):
+    import torch
+
+    k0, out, fields, counters, _ = _steps(SLAB_MEMBERS, monkeypatch, True)
+    alone, _ = _make(SLAB_MEMBERS)
+    for i, c in enumerate(alone):
+        c.langevin_init(np.random.default_rng(98).normal(0, 1.0, c.M))
+        ka, _ = c.anderson_mix(200, 1e-4)
+        assert ka == k0[i]
+        for s in range(2):
+            e = torch.from_numpy(np.random.default_rng(50 * s + i).normal(0, 1.0, c.M)).cuda()
+            torch.cuda.synchronize()
+            lnq, k = c.full_step_dev(e.data_ptr(), 12.0 + i, 0.5)
+            assert lnq == out[s][0][i] and k == out[s][1][i], (i, s)
+        assert np.array_equal(c.get_fields(4), fields[i]), i
+        assert c.counters() == counters[i]
+        c.close()
