@@ -179,13 +179,25 @@ int lfts_batch_langevin_step_dev(lfts_batch* b, const double* const* eta_new_dev
                                  const double* dt);
 int lfts_batch_full_step_dev(lfts_batch* b, const double* const* eta_new_dev, const double* XbN, const double* dt,
                              int maxIter, double errTol, double* lnQ, int* k_ret);
-/* Autotuner report (no reference counterpart): the contour-step kernel shape chosen for
- * each mesh size and launch size, with the measured time per pass of every shape. The
- * shapes differ in where the twiddle/kernel tables live (registers or LDS) and in how
- * many fields a workgroup serves; all give bit-identical results. Set LFTS_SLAB_VARIANT
- * to 0..3 to fix the shape instead. The text stays valid until the next call on the set. */
+/* Autotuner report (no reference counterpart, in place of FFTW_PATIENT planning,
+ * src/step.cc:15-16): the contour-step kernel shape chosen for each mesh size, launch
+ * size and pass kind (with or without fused density sums), with the measured time per
+ * pass of every candidate. A shape is a line factorisation R1 x R2 (R2 threads holding R1
+ * elements of a line) and a variant: twiddle/kernel tables in registers or LDS, one or two
+ * fields per workgroup. Variants of one factorisation give bit-identical results. Set
+ * LFTS_SLAB_VARIANT to 0..3 to fix the variant, LFTS_SLAB_SHAPES the factorisations. The
+ * text stays valid until the next call on the set. */
 const char* lfts_batch_tuning(const lfts_batch* b);
 const char* lfts_tuning(const lfts_ctx* ctx);
+/* The line factorisations R1 x R2 of the contour-step kernel chosen so far in this process,
+ * one per mesh edge, as text ("64:8x8,48:12x4"); with buf = NULL *len receives the size
+ * needed (no reference counterpart: the reference's FFTW plans by timing, src/step.cc:15-16,
+ * and keeps no wisdom). A factorisation is chosen by timing at an edge's first use and then
+ * shared by every context and set of the process; different factorisations round
+ * differently (about 1 ulp per transform), so a run that must repeat another's bits sets
+ * the environment variable LFTS_SLAB_SHAPES to this text (or "default") before its first
+ * evaluation. */
+int lfts_slab_shapes(char* buf, long long cap, long long* len);
 /* lfts_batch_full_step_dev with host noise: member id's M doubles eta_new[id] are copied
  * to its device staging buffer first (as lfts_langevin_step does for one context). */
 int lfts_batch_full_step(lfts_batch* b, const double* const* eta_new, const double* XbN, const double* dt,

@@ -82,6 +82,7 @@ SIGNATURES = {
     "lfts_sk_reset": (ctypes.c_int, [_vp]),
     "lfts_sk_bins": (ctypes.c_int, [_ip, _dp, _dp, _ip, _ip]),
     "lfts_lu_solve": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp]),
+    "lfts_slab_shapes": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]),
     "lfts_step_fwd_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
     "lfts_sk_save": (ctypes.c_int, [ctypes.c_char_p, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_int]),
@@ -469,6 +470,15 @@ class Rng:
         out = np.zeros(int(n))
         lib().lfts_rng_uniforms(self.h, _d(out), int(n))
         return out
+
+
+def slab_shapes() -> str:
+    """The contour-step line factorisations chosen in this process so far, e.g. "64:8x8"."""
+    n = ctypes.c_longlong()
+    lib().lfts_slab_shapes(None, 0, ctypes.byref(n))
+    buf = ctypes.create_string_buffer(max(n.value, 1))
+    lib().lfts_slab_shapes(buf, n.value, ctypes.byref(n))
+    return buf.raw[:n.value].decode()
 
 
 def lu_solve(A: np.ndarray, y: np.ndarray) -> np.ndarray:

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 LIB_SOURCES = ["lfts_kernels.hip", "lfts_fused.hip", "lfts_batch.hip", "lfts_abi.hip", "lfts_adjoint.hip", "lfts_ckpt.hip",
-               "lfts_noalloc.hip"]
+               "lfts_noalloc.hip", "lfts_slab_alt.hip"]
 DRIVER_SOURCES = ["lfts_params.cc", "lfts_simulation.cc", "fts_gpu.cc"]
 
 

@@ -14,6 +14,7 @@
 #include <cstddef>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <sstream>
@@ -302,65 +303,165 @@ int read_scal(lfts_batch* b) {
 
 double scal(const lfts_batch* b, int id, int k) { return b->h_scal[(size_t)id * lfts::SSTRIDE + k]; }
 
-// The slab kernel variant (*variant) for `cnt` replicas of edge `edge` (descriptors
-// d_reps, host copies h_reps), chosen once per set and size class by timing each variant on contour-step passes 1
-// and 2 after a prologue pass (their outputs are rewritten by the next chain). The
-// environment variable LFTS_SLAB_VARIANT = 0..3 fixes the choice instead.
+// ---- kernel-shape autotuner ------------------------------------------------------------
+// A slab kernel shape is a line factorisation R1 x R2 (lfts::slab_shape_factors) with a
+// variant: tables in registers or LDS, one or two fields per workgroup. Variants of one
+// factorisation give bit-identical results; factorisations round differently. So the
+// factorisation of an edge is chosen once per process, when the edge is first tuned (like
+// a plan by timing that every later use of the size shares), and every context and set of
+// the process then rounds alike: a member gives the same bits alone and in any set. The
+// variant is chosen per (edge, launch-size class, fused density sums or not).
+// LFTS_SLAB_SHAPES fixes factorisations ("default", or e.g. "64:16x4,48:6x8"),
+// LFTS_SLAB_VARIANT = 0..3 the variant.
+namespace {
+std::mutex g_shape_mu;
+std::map<int, int> g_shape;  // edge -> factorisation chosen by timing in this process
+std::map<int, int> g_used;   // edge -> factorisation last used (chosen or fixed)
+
+// the factorisation LFTS_SLAB_SHAPES fixes for this edge, or -1
+int env_shape(int edge) {
+    const char* e = std::getenv("LFTS_SLAB_SHAPES");
+    if (!e || !*e) return -1;
+    if (std::strcmp(e, "default") == 0) return 0;
+    const std::string spec(e);
+    size_t pos = 0;
+    while (pos < spec.size()) {
+        size_t end = spec.find(',', pos);
+        if (end == std::string::npos) end = spec.size();
+        int E = 0, r1 = 0, r2 = 0;
+        if (std::sscanf(spec.substr(pos, end - pos).c_str(), "%d:%dx%d", &E, &r1, &r2) == 3 && E == edge)
+            for (int sh = 0; sh < lfts::slab_shape_count(edge); sh++) {
+                int a = 0, b2 = 0;
+                if (lfts::slab_shape_factors(edge, sh, &a, &b2) && a == r1 && b2 == r2) return sh;
+            }
+        pos = end + 1;
+    }
+    return -1;
+}
+
+std::string shape_name(int edge, int shape) {
+    int r1 = 0, r2 = 0;
+    lfts::slab_shape_factors(edge, shape, &r1, &r2);
+    return std::to_string(r1) + "x" + std::to_string(r2);
+}
+}  // namespace
+
+// The slab kernel shape (*variant = 4 x factorisation + variant) for `cnt` replicas of edge
+// `edge` (descriptors d_reps, host copies h_reps) and passes with fused density sums
+// (fuse = 1, one replica) or without. Timed on two contour-step passes after a prologue
+// (with fused sums: the two passes from the first one that fuses): three rounds over the
+// candidates, the fastest round of each counting. An alternative factorisation replaces
+// an edge's default only when it is at least 3 % faster, so that near ties keep the
+// default in every run. The tuning passes write the pairs of their beads, the partially
+// convolved pairs T0/T1 and, fused, the running density sums: the chain that follows
+// rewrites all of them before it reads them.
 int slab_variant(lfts_batch* b, int edge, int cnt, const Rep* d_reps, const Rep* h_reps, const double2* tw,
-                 hipStream_t s, int* variant) {
+                 hipStream_t s, int* variant, int fuse) {
     // launch sizes are tuned in power-of-two classes (the active set shrinks one member
     // at a time as members converge)
     int cls = 1;
     while (cls < cnt) cls *= 2;
-    const auto key = std::make_pair(edge, cls);
+    const auto key = std::make_tuple(edge, cls, fuse);
     const auto it = b->tuned.find(key);
     if (it != b->tuned.end()) {
         *variant = it->second;
         return LFTS_OK;
     }
-    int best = 0;
+    const int V = lfts::SLAB_VARIANTS;
+    const int nsh = lfts::slab_shape_count(edge);
+    // LFTS_SLAB_SHAPES first, then the process's earlier choice, else all candidates
+    const int env_sh = env_shape(edge);
+    int fixed_sh = env_sh;
+    if (fixed_sh < 0) {
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        const auto g = g_shape.find(edge);
+        if (g != g_shape.end()) fixed_sh = g->second;
+    }
     const char* env = std::getenv("LFTS_SLAB_VARIANT");
-    char line[160];
-    if (edge >= 72) {  // one shape exists at these sizes (LDS tables, one field per workgroup)
-        best = 1;
-        std::snprintf(line, sizeof line, "edge %d x %d: variant 1 (the only shape)\n", edge, cls);
-    } else if (env && *env >= '0' && *env < '0' + lfts::SLAB_VARIANTS && env[1] == 0) {
-        best = *env - '0';
-        std::snprintf(line, sizeof line, "edge %d x %d: variant %d (fixed)\n", edge, cls, best);
-    } else {
+    const int fixed_v = (env && *env >= '0' && *env < '0' + V && env[1] == 0) ? *env - '0' : -1;
+    // candidates: (factorisation, variant)
+    std::vector<std::pair<int, int>> cand;
+    for (int sh = 0; sh < nsh; sh++) {
+        if (fixed_sh >= 0 && sh != fixed_sh) continue;
+        const int mask = lfts::slab_shape_variants(edge, sh);
+        for (int v = 0; v < V; v++)
+            if (((mask >> v) & 1) && (fixed_v < 0 || v == fixed_v || !((mask >> fixed_v) & 1))) cand.push_back({sh, v});
+    }
+    if (cand.empty()) return bfail(b, LFTS_EINVAL, "slab_variant: no kernel shape for this edge");
+    const Rep* one = (cnt == 1) ? h_reps : nullptr;
+    int i1 = 1;  // first timed pass
+    if (fuse && one) i1 = (one->N + 1) / 2;
+    std::vector<float> t(cand.size(), 1e30f);
+    if (cand.size() > 1) {
         for (hipEvent_t& e : b->tune_ev)
             if (!e) BHIPCHK(b, hipEventCreate(&e));
-        const Rep* one = (cnt == 1) ? h_reps : nullptr;
         BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 0, tw, s, one, 0));
-        // three rounds over the variants in turn, the fastest round of each counts
-        float t[lfts::SLAB_VARIANTS];
-        for (int v = 0; v < lfts::SLAB_VARIANTS; v++) t[v] = 1e30f;
         for (int round = 0; round < 3; round++)
-            for (int v = 0; v < lfts::SLAB_VARIANTS; v++) {
-                BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 1, tw, s, one, v));  // warm-up
-                BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 2, tw, s, one, v));
+            for (size_t c = 0; c < cand.size(); c++) {
+                const int v = cand[c].first * V + cand[c].second;
+                BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, i1, tw, s, one, v, fuse));  // warm-up
+                BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, i1 + 1, tw, s, one, v, fuse));
                 BHIPCHK(b, hipEventRecord(b->tune_ev[0], s));
                 for (int rep = 0; rep < 4; rep++) {
-                    BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 1, tw, s, one, v));
-                    BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, 2, tw, s, one, v));
+                    BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, i1, tw, s, one, v, fuse));
+                    BHIPCHK(b, lfts::launch_slab(d_reps, cnt, edge, i1 + 1, tw, s, one, v, fuse));
                 }
                 BHIPCHK(b, hipEventRecord(b->tune_ev[1], s));
                 BHIPCHK(b, hipEventSynchronize(b->tune_ev[1]));
                 float ms = 0.f;
                 BHIPCHK(b, hipEventElapsedTime(&ms, b->tune_ev[0], b->tune_ev[1]));
-                t[v] = std::min(t[v], ms);
+                t[c] = std::min(t[c], ms);
             }
-        for (int v = 1; v < lfts::SLAB_VARIANTS; v++)
-            if (t[v] < t[best]) best = v;
-        std::snprintf(line, sizeof line, "edge %d x %d (tuned on %d): variant %d (us per pass: %.2f %.2f %.2f %.2f)\n",
-                      edge, cls, cnt, best, 125.0 * t[0], 125.0 * t[1], 125.0 * t[2], 125.0 * t[3]);
     }
-    b->tune_report += line;
+    // fastest variant of each factorisation, then the factorisation
+    std::vector<int> bestc(nsh, -1);
+    for (size_t c = 0; c < cand.size(); c++) {
+        int& bc = bestc[cand[c].first];
+        if (bc < 0 || t[c] < t[bc]) bc = (int)c;
+    }
+    int sh = fixed_sh;
+    if (env_sh < 0) {
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        const auto g = g_shape.find(edge);
+        if (g != g_shape.end()) sh = g->second;  // chosen meanwhile by another set
+        if (sh < 0) {
+            sh = 0;
+            for (int k = 1; k < nsh; k++)
+                if (bestc[k] >= 0 && (bestc[sh] < 0 || t[bestc[k]] < 0.97f * t[bestc[sh]])) sh = k;
+        }
+        g_shape[edge] = sh;
+    }
+    int bc = bestc[sh];
+    if (bc < 0) {  // the factorisation another set chose was not among these candidates
+        for (size_t c = 0; c < cand.size(); c++)
+            if (cand[c].first == sh) bc = (int)c;
+        if (bc < 0) return bfail(b, LFTS_EINVAL, "slab_variant: no variant of the chosen factorisation");
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        g_used[edge] = sh;
+    }
+    const int best = cand[bc].first * V + cand[bc].second;
+    std::string line = "edge " + std::to_string(edge) + " x " + std::to_string(cls) + (fuse ? " fused" : "") +
+                       " (tuned on " + std::to_string(cnt) + "): " + shape_name(edge, sh) + " variant " +
+                       std::to_string(cand[bc].second);
+    if (cand.size() > 1) {
+        line += " (us per pass:";
+        char buf[64];
+        for (size_t c = 0; c < cand.size(); c++) {
+            std::snprintf(buf, sizeof buf, " %s/%d %.2f", shape_name(edge, cand[c].first).c_str(), cand[c].second,
+                          125.0 * t[c]);
+            line += buf;
+        }
+        line += ")";
+    } else {
+        line += (fixed_sh >= 0 || fixed_v >= 0) ? " (fixed)" : " (the only shape)";
+    }
+    b->tune_report += line + "\n";
     b->tuned[key] = best;
     *variant = best;
     return LFTS_OK;
 }
-
 // One propagator evaluation (src/diblock.cc:30-60) of every active replica; Q of
 // replica id lands in d_scal[id][0].
 int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int part, long long* npasses_out) {
@@ -394,11 +495,16 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int 
     const int ng = (int)b->groups.size();
     const bool fork = ng > 1;
     // kernel shapes, tuned (on first use of a size class) before the groups run concurrently
-    int variant[lfts::N_SLAB_EDGES] = {};
+    int variant[lfts::N_SLAB_EDGES] = {}, variant_fused[lfts::N_SLAB_EDGES] = {};
     for (int g = 0; g < ng; g++) {
         const auto& G = b->groups[g];
-        if (int rc = slab_variant(b, G.edge, G.cnt, b->d_ract + G.off, b->h_ract + G.off, G.tw, b->st, &variant[g]))
+        if (int rc = slab_variant(b, G.edge, G.cnt, b->d_ract + G.off, b->h_ract + G.off, G.tw, b->st, &variant[g], 0))
             return rc;
+        variant_fused[g] = variant[g];
+        if (fuse && G.cnt == 1 && lfts::fused_phi(b->h_ract[G.off], fuse))
+            if (int rc = slab_variant(b, G.edge, 1, b->d_ract + G.off, b->h_ract + G.off, G.tw, b->st,
+                                      &variant_fused[g], 1))
+                return rc;
     }
     if (fork) {
         BHIPCHK(b, hipEventRecord(b->ev_fork, b->st));
@@ -409,8 +515,9 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int 
         hipStream_t s = fork ? b->gst[g] : b->st;
         int Ng = 0;
         for (int k = 0; k < G.cnt; k++) Ng = std::max(Ng, b->ctx[b->act[G.off + k]]->N);
-        const int v = variant[g];
+        const int i0 = (Ng + 1) / 2;  // the first pass that fuses (one replica: Ng is its N)
         for (int i = 0; i < Ng; i++) {
+            const int v = (fuse && i >= i0) ? variant_fused[g] : variant[g];
             BHIPCHK(b, lfts::launch_slab(b->d_ract + G.off, G.cnt, G.edge, i, G.tw, s, b->h_ract + G.off, v, fuse));
             if (i > 0) npasses++;
         }
@@ -1339,6 +1446,23 @@ int lfts_batch_full_step_dev(lfts_batch* b, const double* const* eta_new_dev, co
 }
 
 const char* lfts_batch_tuning(const lfts_batch* b) { return b ? b->tune_report.c_str() : ""; }
+
+int lfts_slab_shapes(char* buf, long long cap, long long* len) {
+    if (!len) return LFTS_EINVAL;
+    std::string t;
+    {
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        for (const auto& kv : g_used) {
+            if (!t.empty()) t += ",";
+            t += std::to_string(kv.first) + ":" + shape_name(kv.first, kv.second);
+        }
+    }
+    *len = (long long)t.size();
+    if (!buf) return LFTS_OK;
+    if (cap < (long long)t.size()) return LFTS_EINVAL;
+    std::memcpy(buf, t.data(), t.size());
+    return LFTS_OK;
+}
 
 const char* lfts_tuning(const lfts_ctx* c) { return (c && c->self) ? c->self->tune_report.c_str() : ""; }
 

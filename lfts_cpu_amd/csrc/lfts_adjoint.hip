@@ -392,8 +392,10 @@ int run_lockstep(lfts_ctx* c, int i0, int i1, ChainSpec* const* chains, int n) {
     if (c->fused) {
         // the kernel shape the context's own plain chain was tuned to (every shape gives
         // the same bits)
-        const auto tv = c->self->tuned.find(std::make_pair(c->d.m[0], 1));
-        const int variant = (tv != c->self->tuned.end()) ? tv->second : 0;
+        // the tuned variant of the plain chain, on the default factorisation (the derivative
+        // chains have no other)
+        const auto tv = c->self->tuned.find(std::make_tuple(c->d.m[0], 1, 0));
+        const int variant = (tv != c->self->tuned.end()) ? tv->second % lfts::SLAB_VARIANTS : 0;
         for (int p = i0 - 1; p < i1; p++)
             for (int k = 0; k < n; k++) {
                 const ChainSpec& ch = *chains[k];

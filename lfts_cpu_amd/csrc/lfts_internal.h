@@ -198,6 +198,17 @@ constexpr int N_SLAB_EDGES = 13;
 // timing (the autotuner of the replica sets). Edges from 72 up have variant 1 only (the
 // other shapes spill registers or exceed the LDS).
 constexpr int SLAB_VARIANTS = 4;
+// Line factorisations R1 x R2 of the slab pass: shape 0 is each edge's default (every chain
+// kind, lfts_fused.hip); shapes 1 and up are the alternatives the autotuner also times for
+// plain chains (lfts_slab_alt.hip). A variant index carries the shape: variant = 4 shape + v.
+// Different factorisations round differently (about 1 ulp per transform), so a process
+// uses one factorisation per edge (see slab_variant).
+constexpr int MAX_SLAB_SHAPES = 3;
+int slab_shape_count(int edge);
+bool slab_shape_factors(int edge, int shape, int* r1, int* r2);
+int slab_shape_variants(int edge, int shape);  // bit v set: variant v exists for this shape
+hipError_t dispatch_edge_alt(int n, int shape, int v, const Rep* reps, int nact, int i, const SlabIO* io,
+                             const double2* tw, hipStream_t st);
 // fuse = 1: density sums fused into the second half of the chain (fused_phi); only for
 // one replica (nact == 1 with its host descriptor `one`).
 hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* tw, hipStream_t st,

@@ -4,6 +4,7 @@
 
 #include <functional>
 #include <map>
+#include <tuple>
 #include <string>
 #include <utility>
 #include <vector>
@@ -112,7 +113,7 @@ struct lfts_batch {
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join[lfts::N_SLAB_EDGES] = {};
     // slab kernel variant per (mesh edge, members in the launch), picked by timing
-    std::map<std::pair<int, int>, int> tuned;
+    std::map<std::tuple<int, int, int>, int> tuned;  // (edge, launch-size class, fused) -> shape
     std::string tune_report;
     hipEvent_t tune_ev[2] = {nullptr, nullptr};
     // live event timing
@@ -179,7 +180,7 @@ enum { PART_ALL = 0, PART_CHAIN = 1, PART_PHI = 2 };
 int calc_concs_async(lfts_batch* b, bool store_all = false, bool h_ready = false, int dots = DOTS_NONE,
                      int part = PART_ALL, long long* npasses_out = nullptr);
 int slab_variant(lfts_batch* b, int edge, int cnt, const lfts::Rep* d_reps, const lfts::Rep* h_reps,
-                 const double2* tw, hipStream_t s, int* variant);
+                 const double2* tw, hipStream_t s, int* variant, int fuse = 0);
 int read_scal(lfts_batch* b);
 int sync_batch(lfts_batch* b);
 std::vector<int> all_ids(const lfts_batch* b);

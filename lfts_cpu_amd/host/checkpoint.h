@@ -8,8 +8,13 @@
 // its sample count, the loop position and the counters. A run restarted from it prints
 // and writes exactly what the uninterrupted run would have.
 //
-// Layout (native byte order): magic "LFTSCKP1", then the fields below in order, then a
+// It also holds the contour-step line factorisations the library's autotuner chose
+// (lfts_slab_shapes): a restart fixes them (LFTS_SLAB_SHAPES) before its first
+// evaluation, so that it rounds exactly as the run it continues.
+//
+// Layout (native byte order): magic "LFTSCKP2", then the fields below in order, then a
 // 64-bit FNV-1a checksum of every preceding byte. Written to <file>.tmp and renamed.
+// Files of the first layout ("LFTSCKP1", no factorisations) still load.
 #pragma once
 #include <cstdint>
 #include <cstdio>
@@ -35,6 +40,7 @@ struct State {
     std::vector<double> S;         // Mk
     int32_t nsamples = 0;
     std::string rng;               // lfts_rng_save text
+    std::string shapes;            // lfts_slab_shapes text (may be empty)
 };
 
 inline uint64_t fnv1a(const unsigned char* p, size_t n, uint64_t h = 1469598103934665603ULL) {
@@ -109,7 +115,7 @@ class Reader {
 
 inline bool save(const std::string& file, const State& s) {
     Writer w;
-    w.raw("LFTSCKP1", 8);
+    w.raw("LFTSCKP2", 8);
     w.put(s.phase);
     w.put(s.it);
     w.put(s.steps_total);
@@ -126,6 +132,8 @@ inline bool save(const std::string& file, const State& s) {
     w.put(s.nsamples);
     w.put<int64_t>((int64_t)s.rng.size());
     w.raw(s.rng.data(), s.rng.size());
+    w.put<int64_t>((int64_t)s.shapes.size());
+    w.raw(s.shapes.data(), s.shapes.size());
     return w.save(file);
 }
 
@@ -133,7 +141,8 @@ inline bool load(const std::string& file, State& s) {
     Reader r(file);
     char magic[8];
     r.raw(magic, 8);
-    if (!r.ok() || std::memcmp(magic, "LFTSCKP1", 8) != 0) return false;
+    const bool v2 = std::memcmp(magic, "LFTSCKP2", 8) == 0;
+    if (!r.ok() || (!v2 && std::memcmp(magic, "LFTSCKP1", 8) != 0)) return false;
     s.phase = r.get<int32_t>();
     s.it = r.get<int64_t>();
     s.steps_total = r.get<int64_t>();
@@ -152,6 +161,13 @@ inline bool load(const std::string& file, State& s) {
     if (!r.ok() || n <= 0 || n > (1 << 20)) return false;
     s.rng.resize((size_t)n);
     r.raw(&s.rng[0], (size_t)n);
+    s.shapes.clear();
+    if (v2) {
+        const int64_t ns = r.get<int64_t>();
+        if (!r.ok() || ns < 0 || ns > 4096) return false;
+        s.shapes.resize((size_t)ns);
+        if (ns) r.raw(&s.shapes[0], (size_t)ns);
+    }
     return r.ok();
 }
 

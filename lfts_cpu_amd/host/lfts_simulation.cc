@@ -117,6 +117,9 @@ void lfts_simulation::write_checkpoint(int phase, long long it) {
     lfts_rng_save(rng_, nullptr, 0, &n);
     st.rng.resize((size_t)n);
     lfts_rng_save(rng_, &st.rng[0], n, &n);
+    lfts_slab_shapes(nullptr, 0, &n);
+    st.shapes.resize((size_t)n);
+    if (n) lfts_slab_shapes(&st.shapes[0], n, &n);
     if (!lfts_ckpt::save(opt_.checkpoint, st)) {
         std::cout << "ERROR => cannot write the checkpoint " << opt_.checkpoint << std::endl;
         std::exit(1);
@@ -141,6 +144,8 @@ void lfts_simulation::restore_checkpoint(const std::string& file) {
         std::cout << "ERROR => the checkpoint's random-stream state is unreadable" << std::endl;
         std::exit(1);
     }
+    // the factorisations the stopped run used (fixed before this run's first evaluation)
+    if (!st.shapes.empty()) setenv("LFTS_SLAB_SHAPES", st.shapes.c_str(), 1);
     check(lfts_langevin_init(ctx_, st.eta_prev.data()), "restart: noise");
     w_ = st.w;
     check(lfts_set_fields(ctx_, w_.data(), 4), "restart: fields");

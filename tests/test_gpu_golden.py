@@ -8,19 +8,20 @@ the Langevin update, Anderson mix to 1e-4, calc_concs and, in the statistics loo
 S(k) sample.
 
 Bars (FP64; relative max-norm per field, or per field sum, sum of squares and 64 sampled
-values): Anderson counts equal; lnQ and H 1e-9; W- 1e-8; phi-, phi+ 5e-8; w+ 1e-5; S(k)
-1e-8. These are trajectory bars. Each run starts with Anderson solves from a random field
-at grid scale, and a solve fixes w+ only to its residual tolerance (1e-4 in phi+), and
-the mean of w+ not at all (a uniform shift of w+ leaves phi unchanged). The solve's
-ill-conditioned history solve turns the last-bit differences of the evaluations into a
-different point of that tolerance ball: after the 128^3 initial mix w+ differs by 2e-6
-and phi+ by 1.3e-8 (5000 times below the residual tolerance), while lnQ agrees to 1.2e-10
-and H to 3e-12; two steps later every field but the mean of w+ agrees to 2e-10. One
-evaluation on identical fields agrees to 1e-10 and better (test_gpu_parity), and the
-100-step test holds Q, phi+-, H and S(k) to 1e-10 after its first step and 1e-8 over all
-100. The `lfts-gpu`
-driver on the 16^3 input prints the fixture's lnQ and H to their 6 printed digits and
-writes the final fields to the 6 digits of its text files."""
+values). Anderson counts must be equal. Every other quantity is held to 1e-8 (the bar of a
+run), unless the oracle run against itself in another rounding order (its 3D transforms
+along x, y, z, its reductions summed from the last point) already drifts further on the
+same run (tests/golden/spread.json, written by tests/golden/make_golden.py spread): then
+the bar is ten times that measured spread. These runs start with an Anderson solve from a
+grid-scale random field, which fixes w+ only to its residual tolerance (1e-4 in phi+) and
+the mean of w+ not at all, and its ill-conditioned history solve turns last-bit
+differences into another point of that tolerance ball: measured oracle against oracle
+after the initial mix, w+ differs by 2.7e-12 at 16^3, 2.1e-10 at 40^3 and 1.3e-6 for a
+48^3 member of the batched workload, and phi by up to 3.5e-9; a few steps later the
+differences shrink by orders of magnitude. The `lfts-gpu` driver on the 16^3 input prints
+the fixture's lnQ and H to their 6 printed digits and writes the final fields to the 6
+digits of its text files."""
+import json
 import math
 import os
 import subprocess
@@ -33,6 +34,16 @@ import lfts_cpu_amd as L
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SPREAD = json.load(open(os.path.join(GOLD, "spread.json")))
+
+
+def bar(name, key, base=1e-8):
+    """Bars from the measured oracle-against-oracle spread of run `name`: a number for a
+    scalar key, a list of four (W-, w+, phi-, phi+) for a field stage."""
+    sp = SPREAD[name][key]
+    if isinstance(sp, list):
+        return [max(base, 10.0 * x) for x in sp]
+    return max(base, 10.0 * sp)
 
 
 def _header(name):
@@ -102,18 +113,20 @@ def test_reference_input_run_matches_fixture(gpu, name):
     assert nam == g["nAM"].tolist()
     d_lnq = max(abs(a - b) / max(1.0, abs(b)) for a, b in zip(lnq, g["lnQ"]))
     d_H = abs(H - float(g["H"])) / max(1.0, abs(float(g["H"])))
-    print(f"{name}: lnQ within {d_lnq:.1e}, H within {d_H:.1e}", flush=True)
-    assert d_lnq < 1e-9 and d_H < 1e-9, (lnq, g["lnQ"], H, float(g["H"]))
-    diffs = [_summary_diffs(g, "init_", w0, M)]
+    print(f"{name}: lnQ within {d_lnq:.1e} (bar {bar(name, 'lnQ'):.1e}), H within {d_H:.1e} "
+          f"(bar {bar(name, 'H'):.1e})", flush=True)
+    assert d_lnq <= bar(name, "lnQ") and d_H <= bar(name, "H"), (lnq, g["lnQ"], H, float(g["H"]))
+    diffs = {"init": _summary_diffs(g, "init_", w0, M)}
     if "w_end" in g:
-        diffs.append([rel(w_end[k * M:(k + 1) * M], g["w_end"][k * M:(k + 1) * M]) for k in range(4)])
+        diffs["end"] = [rel(w_end[k * M:(k + 1) * M], g["w_end"][k * M:(k + 1) * M]) for k in range(4)]
         assert ns == int(g["ns"])
-        assert rel(S, g["sk"]) < 1e-8
+        assert rel(S, g["sk"]) <= bar(name, "sk")
     else:
-        diffs.append(_summary_diffs(g, "end_", w_end, M))
+        diffs["end"] = _summary_diffs(g, "end_", w_end, M)
     print(f"{name}: field differences (W-, w+, phi-, phi+) {diffs}", flush=True)
-    for d in diffs:
-        assert d[0] < 1e-8 and d[1] < 1e-5 and d[2] < 5e-8 and d[3] < 5e-8, d
+    for stage, d in diffs.items():
+        for q, (e, b) in enumerate(zip(d, bar(name, stage))):
+            assert e <= b, (stage, ["W-", "w+", "phi-", "phi+"][q], e, b)
     print(f"{name}: n_AM {k0} {nam}, lnQ {lnq}, H {H!r}", flush=True)
 
 

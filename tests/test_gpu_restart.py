@@ -27,9 +27,16 @@ def _input(tmp_path):
     return p
 
 
-def _run(args, cwd):
+# Separate processes round alike only with the same contour-step factorisations: the
+# library picks one per mesh edge by timing (lfts_slab_shapes), so the runs compared
+# here fix them.
+ENV = dict(os.environ, LFTS_SLAB_SHAPES="default")
+
+
+def _run(args, cwd, env=ENV):
     print("driver", args, flush=True)
-    r = subprocess.run([L.DRIVER_PATH, *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=300)
+    r = subprocess.run([L.DRIVER_PATH, *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=300,
+                       env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     return r.stdout
 
@@ -102,7 +109,7 @@ def test_replica_set_driver_matches_single_runs(gpu, tmp_path):
         singles.append(_run([s / "input"], s))
     print("batch driver", flush=True)
     rb = subprocess.run([L.BATCH_DRIVER_PATH, *map(str, inputs)], cwd=tmp_path, capture_output=True, text=True,
-                        timeout=300)
+                        timeout=300, env=ENV)
     assert rb.returncode == 0, rb.stdout + rb.stderr
     for r in range(len(specs)):
         lb = [ln.split("] ", 1)[1] for ln in rb.stdout.splitlines() if ln.startswith(f"[{r}] lnQ = ")]
@@ -114,3 +121,29 @@ def test_replica_set_driver_matches_single_runs(gpu, tmp_path):
         assert sorted(os.listdir(db)) == files
         for f in files:
             assert (da / f).read_bytes() == (db / f).read_bytes(), (r, f)
+
+
+def test_restart_keeps_the_stopped_runs_factorisation(gpu, tmp_path):
+    """A checkpoint carries the contour-step factorisation its run used: the restart, given
+    no LFTS_SLAB_SHAPES of its own, continues with it (and writes it into its own
+    checkpoints), and prints and writes what the uninterrupted run does."""
+    M = 24 ** 3
+    rng = np.random.default_rng(33)
+    w0 = np.concatenate([rng.normal(0, 0.1, M), rng.normal(0, 0.1, M)])
+    text = "12 5 14.0 50 0.5 1\n24 24 24 3.4 3.4 3.4\n4 4 2 2 1\n" + "".join("%.17g\n" % v for v in w0)
+    alt = dict(os.environ, LFTS_SLAB_SHAPES="24:6x4")
+    free = {k: v for k, v in os.environ.items() if k != "LFTS_SLAB_SHAPES"}
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    (a / "input").write_text(text)
+    (b / "input").write_text(text)
+    full = _run([a / "input"], a, alt)
+    first = _run([b / "input", "--checkpoint", b / "ck", "--stop-after", 3], b, alt)
+    assert b"24:6x4" in (b / "ck").read_bytes()
+    rest = _run([b / "input", "--restart", b / "ck", "--checkpoint", b / "ck2", "--checkpoint-every", 1], b, free)
+    assert b"24:6x4" in (b / "ck2").read_bytes()
+    assert _lnq(first) + _lnq(rest) == _lnq(full)
+    for f in sorted(os.listdir(a)):
+        if f.startswith(("w_", "phi_", "struct_")):
+            assert (a / f).read_bytes() == (b / f).read_bytes(), f
