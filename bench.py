@@ -349,7 +349,7 @@ def main():
     ap.add_argument("--dims", type=int, nargs=3, default=None,
                     help="a non-cubic mesh mx my mz instead of --mesh (same workload otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--batched-steps", type=int, default=3, help="timed steps of the replica-set leg (0 skips it)")
+    ap.add_argument("--batched-steps", type=int, default=5, help="timed steps of the replica-set leg (0 skips it)")
     ap.add_argument("--batched-quick", action="store_true", help="8 members instead of 64 (smoke runs)")
     ap.add_argument("--only-batched", action="store_true", help="run only the replica-set leg (profiling)")
     ap.add_argument("--batched-no-sequential", action="store_true",
@@ -504,6 +504,8 @@ def main():
     if args.batched_steps > 0:
         ctx.close()
         batched = batched_leg(args.batched_steps, 1, L, torch, quick=args.batched_quick)
+        if batched.get("contour_chain_algorithmic_GBps"):
+            batched["chain_frac_of_attainable"] = batched["contour_chain_algorithmic_GBps"] / copy_gbps
 
     out = {
         "metric": METRIC,
@@ -537,6 +539,7 @@ def main():
                       f"evaluation), HIP events on the context stream over every {PROF_EVERY}th evaluation of the timed steps",
             "contour_step_us": cs_ms * 1e3,
             "attainable_copy_GBps": copy_gbps,
+            "frac_of_attainable": achieved / copy_gbps,
             "slab_tuning": tuning_plain,
         },
         "step_ms_median": float(np.median(step_ms)),
