@@ -186,3 +186,15 @@ def test_lu_solve_rejects_bad_sizes():
     x = np.zeros(16)
     assert L.lib().lfts_lu_solve(16, L._d(np.eye(16).reshape(-1)), L._d(np.ones(16)), L._d(x)) == L.LFTS_EINVAL
     assert L.lib().lfts_lu_solve(0, L._d(np.eye(1).reshape(-1)), L._d(np.ones(1)), L._d(x)) == L.LFTS_EINVAL
+
+
+def test_slab_shapes_text_before_any_tuning():
+    """lfts_slab_shapes reports the factorisations chosen so far: none before a GPU ran."""
+    import ctypes
+
+    n = ctypes.c_longlong(-1)
+    assert L.lib().lfts_slab_shapes(None, 0, ctypes.byref(n)) == L.LFTS_OK
+    assert n.value >= 0
+    assert L.lib().lfts_slab_shapes(None, 0, None) == L.LFTS_EINVAL
+    if not has_gpu():
+        assert L.slab_shapes() == ""
