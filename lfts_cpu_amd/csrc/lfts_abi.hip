@@ -583,6 +583,15 @@ int put_flags(lfts_batch* b, const std::vector<int>& eval, const std::vector<int
     return LFTS_OK;
 }
 
+// Every flag back to 0 (every member evaluated and updated). A fill on the stream, not an
+// upload: the staging buffer may still be read by a copy enqueued just before (a pinned
+// source is read when the copy runs, not when it is enqueued).
+int clear_flags(lfts_batch* b) {
+    const int n = (int)b->ctx.size();
+    BHIPCHK(b, hipMemsetAsync(b->d_flags, 0, sizeof(int) * 2 * n, b->st));
+    return LFTS_OK;
+}
+
 // One Anderson iteration after the first, every member in the launches, in three parts:
 // (0) flags and coefficients (pinned staging, read when the graph runs) and the update
 // with the Boltzmann factors; (1) the contour-step chains on the groups' streams; (2) Q,
@@ -762,6 +771,7 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
                             DD(0, 0) - DD(0, m) - DD(0, nn) + DD(nn, m);
                 }
                 if (!lu_solve(nh, c->U.data(), c->V.data(), c->C.data())) {
+                    if (graph) (void)clear_flags(b);
                     (void)sync_batch(b);
                     return bfail(b, LFTS_ESING, "singular Anderson system: the history matrix U has a zero pivot "
                                                 "(src/anderson.cc:124-125; GSL reports it singular and aborts)");
@@ -802,7 +812,7 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             if (int rc = upload_coef(b)) return rc;
             BHIPCHK(b, lfts::launch_am_update(b->d_rupd, n, b->nb_max, b->d_coef, 1, b->st));
         }
-        if (int rc = put_flags(b, all_ids(b), all_ids(b), true)) return rc;
+        if (int rc = clear_flags(b)) return rc;
     }
     if (int rc = sync_batch(b)) return rc;
     bool nonfinite = false;
