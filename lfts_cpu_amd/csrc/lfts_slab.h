@@ -616,9 +616,28 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 
 }  // namespace
 
-// One slab pass of one replica (plane `plane`), addresses in `io`. The workgroup serves
-// field blockIdx.y, or with FPW = 2 both fields (threads [0, TPF) field 0, [TPF, 2 TPF)
-// field 1). TWL = 1 keeps the twiddle and kernel tables in LDS instead of registers.
+// LDS of one slab pass shape: FPW exchange/plane buffers (the padded exchange rows of the
+// line DFTs, reused for the re-paired real plane) and, with TWL = 1, the tables. The
+// kernel owns the array (a kernel that runs several shapes sizes it for the largest) and
+// hands it to slab_pass.
+template <int E, int R1, int R2, int TWL, int FPW>
+struct SlabLds {
+    static constexpr int H = E / 2;                      // complex lines per plane
+    static constexpr int XP = (R1 * (R2 + 1)) | 1;       // exchange row pitch (complex): padded line, odd
+    static constexpr int RP = E + ((E + 8) % 16 == 8 ? 8 : 16);  // real plane row pitch (doubles), = 8 mod
+                                                                 // 16: rows r, r+1 on other banks
+    static constexpr int XBYTES = H * XP * 16;
+    static constexpr int RBYTES = E * RP * 8;
+    static constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
+    static constexpr int TBYTES = TWL ? E * 16 + 2 * E * 8 : 0;
+    static constexpr int TOTAL = FPW * BYTES + TBYTES;
+    static constexpr int THREADS = H * R2 * FPW;        // workgroup size of the shape
+};
+
+// One slab pass of one replica (plane `plane`), addresses in `io`, LDS `smem` (SlabLds
+// TOTAL bytes). The workgroup serves field `fidx`, or with FPW = 2 both fields (threads
+// [0, TPF) field 0, [TPF, 2 TPF) field 1). TWL = 1 keeps the twiddle and kernel tables in
+// LDS instead of registers.
 // MODE selects the finishing rule of pair i+1 from u = (C x)(r):
 //   CHAIN_PLAIN    out = h u                   (src/step.cc:53-64)
 //   CHAIN_TANGENT  out = h u + aux src         (directional derivative of the chain)
@@ -641,20 +660,17 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
 // members, 122 MB for the 64-member workload, is read back by the next launch, and
 // cached stores made the 64^3 and 48^3 group passes 9-13 % faster).
 template <int E, int R1, int R2, int MODE, int TWL, int FPW, int FUSE, int FIN, bool TNT = true>
-__device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const double2* __restrict__ tw) {
+__device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const double2* __restrict__ tw,
+                                          unsigned char* smem, int fidx) {
     static_assert(R1 * R2 == E, "unsupported factorisation");
-    constexpr int H = E / 2;           // complex lines per plane
+    using Lds = SlabLds<E, R1, R2, TWL, FPW>;
+    constexpr int H = Lds::H;
     constexpr int TPF = H * R2;        // threads per field
-    constexpr int XP = (R1 * (R2 + 1)) | 1;  // exchange row pitch (complex): padded line, odd
-    constexpr int RP = E + ((E + 8) % 16 == 8 ? 8 : 16);  // real plane row pitch (doubles), = 8 mod 16:
-                                                          // rows r, r+1 on other banks
-    constexpr int XBYTES = H * XP * 16;
-    constexpr int RBYTES = E * RP * 8;
-    constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
-    constexpr int TBYTES = TWL ? E * 16 + 2 * E * 8 : 0;
+    constexpr int XP = Lds::XP;
+    constexpr int RP = Lds::RP;
+    constexpr int BYTES = Lds::BYTES;
     constexpr bool EXTRA = MODE != CHAIN_PLAIN;
     constexpr bool PLAIN = MODE == CHAIN_PLAIN && FUSE;
-    __shared__ __align__(16) unsigned char smem[FPW * BYTES + TBYTES];
     if (io.skip && *io.skip) return;
 
     const bool xslab = io.xslab != 0;
@@ -668,7 +684,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
 
     const int fl = (FPW == 2) ? (int)(threadIdx.x / TPF) : 0;
     const int t = threadIdx.x - fl * TPF;
-    const int f = (FPW == 2) ? fl : (int)blockIdx.y;
+    const int f = (FPW == 2) ? fl : fidx;
     double2* xbuf = reinterpret_cast<double2*>(smem + fl * BYTES);
     double* rp = reinterpret_cast<double*>(smem + fl * BYTES);
     const long pbase = (long)plane * plane_stride;  // plane origin within a field
@@ -817,11 +833,12 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
 template <int E, int R1, int R2, int TWL, int FPW, int FIN>
 __global__ __launch_bounds__((E / 2) * R2 * FPW) void k_slab(const Rep* __restrict__ reps,
                                                            const double2* __restrict__ tw, int i) {
+    __shared__ __align__(16) unsigned char smem[SlabLds<E, R1, R2, TWL, FPW>::TOTAL];
     const int slot = blockIdx.x / E, plane = blockIdx.x - slot * E;
     const Rep& R = reps[slot];  // fields read in place (a private copy would live in scratch)
     if (i > R.N - 1) return;    // this replica's chain is complete
     const SlabIO io = plain_io(R, i);
-    slab_pass<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, FIN, false>(io, plane, tw);
+    slab_pass<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, FIN, false>(io, plane, tw, smem, (int)blockIdx.y);
 }
 
 // One replica, its pass addresses in the kernel arguments (no dependent descriptor load
@@ -831,7 +848,8 @@ __global__ __launch_bounds__((E / 2) * R2 * FPW) void k_slab(const Rep* __restri
 template <int E, int R1, int R2, int MODE, int TWL, int FPW, int FUSE, int FIN>
 __global__ __launch_bounds__((E / 2) * R2 * FPW) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_slab_io(const SlabIO io, const double2* __restrict__ tw) {
-    slab_pass<E, R1, R2, MODE, TWL, FPW, FUSE, FIN>(io, blockIdx.x, tw);
+    __shared__ __align__(16) unsigned char smem[SlabLds<E, R1, R2, TWL, FPW>::TOTAL];
+    slab_pass<E, R1, R2, MODE, TWL, FPW, FUSE, FIN>(io, blockIdx.x, tw, smem, (int)blockIdx.y);
 }
 
 namespace {
