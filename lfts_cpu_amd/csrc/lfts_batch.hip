@@ -164,6 +164,9 @@ __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, co
 // NH >= 0: the same pass also forms the Anderson Gram row partials <Dh0, Dh_m>, m = 0..NH
 // (Dh0 = phi+ - 1, src/anderson.cc:35-55) over the same points in the same order as
 // k_am_dots, so the partial rows are identical and the separate pass is saved.
+// a stored pair or running sum, read once per evaluation: non-temporal
+__device__ __forceinline__ double ld_once(const double* p) { return __builtin_nontemporal_load(p); }
+
 template <int NH>
 __device__ __forceinline__ void phi_row(const Rep& R, double Q, int fuse, double* redn, double* out) {
     const long M = R.M;
@@ -189,7 +192,7 @@ __device__ __forceinline__ void phi_row(const Rep& R, double Q, int fuse, double
         if (fused) {  // phacc holds 4 M doubles: load all four sums, keep the used ones
             double v[4];
 #pragma unroll
-            for (int a = 0; a < 4; a++) v[a] = R.phacc[(long)a * M + r];
+            for (int a = 0; a < 4; a++) v[a] = ld_once(R.phacc + (long)a * M + r);
             load_hist();
 #pragma unroll
             for (int a = 0; a < 4; a++) s[a] = u[a] ? v[a] : 0.0;
@@ -207,25 +210,25 @@ __device__ __forceinline__ void phi_row(const Rep& R, double Q, int fuse, double
                 double x[U], y[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    x[u] = rep_pair(R, j + u)[r];              // q_j
-                    y[u] = rep_pair(R, N + 1 - j - u)[M + r];  // q+_j
+                    x[u] = ld_once(rep_pair(R, j + u) + r);              // q_j
+                    y[u] = ld_once(rep_pair(R, N + 1 - j - u) + M + r);  // q+_j
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) add((j + u <= NA) ? 0 : 1, x[u], y[u]);
             }
-            for (; j <= N; j++) add((j <= NA) ? 0 : 1, rep_pair(R, j)[r], rep_pair(R, N + 1 - j)[M + r]);
+            for (; j <= N; j++) add((j <= NA) ? 0 : 1, ld_once(rep_pair(R, j) + r), ld_once(rep_pair(R, N + 1 - j) + M + r));
             j = N - i0;
             for (; j - U + 1 >= 1; j -= U) {
                 double x[U], y[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    x[u] = rep_pair(R, N + 1 - j + u)[M + r];  // q+_j
-                    y[u] = rep_pair(R, j - u)[r];              // q_j
+                    x[u] = ld_once(rep_pair(R, N + 1 - j + u) + M + r);  // q+_j
+                    y[u] = ld_once(rep_pair(R, j - u) + r);              // q_j
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) add((j - u <= NA) ? 2 : 3, x[u], y[u]);
             }
-            for (; j >= 1; j--) add((j <= NA) ? 2 : 3, rep_pair(R, N + 1 - j)[M + r], rep_pair(R, j)[r]);
+            for (; j >= 1; j--) add((j <= NA) ? 2 : 3, ld_once(rep_pair(R, N + 1 - j) + M + r), ld_once(rep_pair(R, j) + r));
             load_hist();
         }
         double pA = s[0] + s[2], pB = s[1] + s[3];
