@@ -340,6 +340,52 @@ def adjoint_leg(ctx, eta_dev, eta_more, XbN, dt, M, torch, tols=(1e-4, 1e-12)):
     }
 
 
+def driver_leg(w0, m, Lb, N, NA, XbN, C, dt, warmup, steps, lnq_last):
+    """The drop-in driver `lfts-gpu` on the timed steps: as host/lfts_simulation.cc runs a
+    step, its noise (M normals of the seed-123456789 stream) is drawn on a host thread while
+    the GPU runs the step before, and handed over in host memory. The workload's start
+    fields go into an input file (loadType 1, 17 significant digits, read back exactly);
+    the driver draws the same stream in the same order (src/lfts_simulation.cc:38,45-48),
+    so it runs the same steps as the timed region. One run of `warmup + steps` steps,
+    timed from the line it prints after step `warmup` to the line after the last step."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "lfts_cpu_amd", "bin", "lfts-gpu")
+    if not os.path.exists(exe):
+        return None
+    n_eq = warmup + steps
+    stamps, lnqs = [], []
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "input")
+        with open(path, "w") as f:
+            f.write(f"{N} {NA} {XbN!r} {C!r} {dt!r} 0\n")
+            f.write(f"{m[0]} {m[1]} {m[2]} {Lb[0]!r} {Lb[1]!r} {Lb[2]!r}\n")
+            f.write(f"{n_eq} 0 1 {10 * n_eq} 1\n")
+            np.savetxt(f, w0, fmt="%.17g")
+        p = subprocess.Popen([exe, path], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        tail = []
+        for line in p.stdout:  # the driver ends every line with std::endl (flushed)
+            if line.startswith("lnQ ="):
+                stamps.append(time.perf_counter())
+                lnqs.append(float(line.split("=")[1]))
+            tail = (tail + [line])[-20:]
+        if p.wait() != 0:
+            raise RuntimeError(f"lfts-gpu failed ({p.returncode}): {''.join(tail)}")
+    if len(lnqs) != n_eq:
+        raise RuntimeError(f"lfts-gpu printed {len(lnqs)} lnQ lines for {n_eq} steps")
+    # the driver prints 6 significant digits: the same trajectory as the timed steps
+    if abs(lnqs[-1] - lnq_last) > 1e-5 * max(1.0, abs(lnq_last)):
+        raise RuntimeError(f"lfts-gpu ended at lnQ = {lnqs[-1]}, the timed steps at {lnq_last}")
+    t = stamps[-1] - stamps[warmup - 1] if warmup > 0 else stamps[-1] - stamps[0]
+    n = steps if warmup > 0 else steps - 1
+    return {
+        "steps_per_s": n / t,
+        "sample": f"lfts-gpu on the workload's start fields for {n_eq} steps, timed between its printed lnQ "
+                  f"lines of steps {max(warmup, 1)} and {n_eq} ({t:.3f} s); its last lnQ {lnqs[-1]} (6 digits) "
+                  f"matches the timed steps'",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -463,8 +509,7 @@ def main():
     log(f"copy bandwidth: {copy_gbps:.0f} GB/s")
 
     # PCIe-inclusive rate (reported, never `value`): the same step with the noise handed
-    # over in host memory, as the lfts-gpu driver does it (its noise generation runs on a
-    # host thread meanwhile; here it is drawn beforehand)
+    # over in host memory, as the lfts-gpu driver does it (drawn beforehand here)
     host_eta = [rng.normals(M) for _ in range(HOST_STEPS)]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -475,30 +520,14 @@ def main():
     torch.cuda.synchronize()
     host_rate = HOST_STEPS / (time.perf_counter() - t1)
     log(f"host-noise steps: {host_rate:.2f} steps/s")
-
-    # driver-style rate (reported, never `value`): as host/lfts_simulation.cc runs it, each
-    # step's M normals are drawn by the host stream (mt19937_64 + normal_distribution) on a
-    # second thread while the GPU runs the step before, then handed over in host memory
-    from concurrent.futures import ThreadPoolExecutor
-
-    with ThreadPoolExecutor(1) as ex:
-        nxt = ex.submit(rng.normals, M)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(HOST_STEPS):
-            e = nxt.result()
-            nxt = ex.submit(rng.normals, M)  # the next step's noise, drawn meanwhile
-            ctx.langevin_step(e, XbN, dt)
-            ctx.anderson_mix(200, 1e-4)
-            ctx.calc_concs()
-        torch.cuda.synchronize()
-        driver_rate = HOST_STEPS / (time.perf_counter() - t1)
-        nxt.result()
     t1 = time.perf_counter()
     rng.normals(M)
     draw_ms = (time.perf_counter() - t1) * 1e3
-    log(f"driver-style steps (noise drawn on a host thread): {driver_rate:.2f} steps/s; one draw of M normals "
-        f"{draw_ms:.1f} ms")
+
+    # the drop-in driver itself on the timed steps (reported, never `value`)
+    driver = None if args.dims else driver_leg(w0, m, Lb, N, NA, XbN, C, dt, args.warmup, args.steps, lnq)
+    driver_rate = driver["steps_per_s"] if driver else None
+    log(f"driver: {driver}; one draw of M normals {draw_ms:.1f} ms")
 
     batched = None
     if args.batched_steps > 0:
@@ -551,6 +580,7 @@ def main():
         "propagator_eval_ms": ms_pe / max(n_pe, 1),
         "pcie_inclusive_steps_per_s": host_rate,
         "driver_style_steps_per_s": driver_rate,
+        "driver_style_sample": driver["sample"] if driver else None,
         "host_noise_draw_ms": draw_ms,
         "cpu_baseline": cpu,
         "cpu_baseline_oracle": cpu_oracle,
