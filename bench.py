@@ -401,6 +401,8 @@ def main():
     ap.add_argument("--batched-no-sequential", action="store_true",
                     help="skip the member-by-member comparison of the replica-set leg (profiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the Newton-Krylov / derivative leg")
+    ap.add_argument("--no-driver", action="store_true",
+                    help="skip the lfts-gpu driver leg (profiling: a profiler would trace the child process too)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -525,7 +527,7 @@ def main():
     draw_ms = (time.perf_counter() - t1) * 1e3
 
     # the drop-in driver itself on the timed steps (reported, never `value`)
-    driver = None if args.dims else driver_leg(w0, m, Lb, N, NA, XbN, C, dt, args.warmup, args.steps, lnq)
+    driver = None if (args.dims or args.no_driver) else driver_leg(w0, m, Lb, N, NA, XbN, C, dt, args.warmup, args.steps, lnq)
     driver_rate = driver["steps_per_s"] if driver else None
     log(f"driver: {driver}; one draw of M normals {draw_ms:.1f} ms")
 
