@@ -527,7 +527,12 @@ def main():
     draw_ms = (time.perf_counter() - t1) * 1e3
 
     # the drop-in driver itself on the timed steps (reported, never `value`)
-    driver = None if (args.dims or args.no_driver) else driver_leg(w0, m, Lb, N, NA, XbN, C, dt, args.warmup, args.steps, lnq)
+    driver = None
+    if not (args.dims or args.no_driver):
+        try:
+            driver = driver_leg(w0, m, Lb, N, NA, XbN, C, dt, args.warmup, args.steps, lnq)
+        except Exception as e:  # a side measurement: report its failure, keep the bench line
+            driver = {"steps_per_s": None, "sample": f"lfts-gpu leg failed: {e}"}
     driver_rate = driver["steps_per_s"] if driver else None
     log(f"driver: {driver}; one draw of M normals {draw_ms:.1f} ms")
 
