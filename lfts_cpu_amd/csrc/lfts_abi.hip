@@ -510,7 +510,34 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int 
         BHIPCHK(b, hipEventRecord(b->ev_fork, b->st));
         for (int g = 0; g < ng; g++) BHIPCHK(b, hipStreamWaitEvent(b->gst[g], b->ev_fork, 0));
     }
+    // A 32^3 group rides in the 64^3 group's launches (one launch per contour step for
+    // both): its passes are latency-bound on their own. Both in their default
+    // factorisations; LFTS_SET_PAIR=0 keeps separate launches.
+    int pa = -1, pb = -1;
+    if (!fuse) {
+        const char* pe = std::getenv("LFTS_SET_PAIR");
+        if (!(pe && std::strcmp(pe, "0") == 0))
+            for (int g = 0; g < ng; g++) {
+                if (variant[g] >= lfts::SLAB_VARIANTS) continue;
+                if (b->groups[g].edge == 64) pa = g;
+                if (b->groups[g].edge == 32) pb = g;
+            }
+        if (pa < 0 || pb < 0 || !lfts::slab_pair_supported(64, 32)) pa = pb = -1;
+    }
+    if (pa >= 0) {
+        const auto& A = b->groups[pa];
+        const auto& B = b->groups[pb];
+        int Na = 0, Nb = 0;
+        for (int k = 0; k < A.cnt; k++) Na = std::max(Na, b->ctx[b->act[A.off + k]]->N);
+        for (int k = 0; k < B.cnt; k++) Nb = std::max(Nb, b->ctx[b->act[B.off + k]]->N);
+        for (int i = 0; i < std::max(Na, Nb); i++) {
+            BHIPCHK(b, lfts::launch_slab_pair(b->d_ract + A.off, A.cnt, A.edge, b->d_ract + B.off, B.cnt, B.edge, i,
+                                              A.tw, B.tw, fork ? b->gst[pa] : b->st));
+            if (i > 0) npasses += (i < Na) + (i < Nb);
+        }
+    }
     for (int g = 0; g < ng; g++) {
+        if (g == pa || g == pb) continue;
         const auto& G = b->groups[g];
         hipStream_t s = fork ? b->gst[g] : b->st;
         int Ng = 0;

@@ -113,6 +113,21 @@ hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* t
     return dispatch_edge(n, variant, reps, nact, i, CHAIN_PLAIN, nullptr, tw, st);
 }
 
+bool slab_pair_supported(int ea, int eb) { return ea == 64 && eb == 32; }
+
+// 64^3 as 8 x 8 with the tables in registers, one field per workgroup, and 32^3 as 4 x 8
+// with the tables in LDS, both fields per workgroup: 256 threads each
+hipError_t launch_slab_pair(const Rep* repsA, int nA, int ea, const Rep* repsB, int nB, int eb, int i,
+                            const double2* twA, const double2* twB, hipStream_t st) {
+    if (!slab_pair_supported(ea, eb) || nA < 1 || nB < 1) return hipErrorInvalidValue;
+    const dim3 grid(nA * 64 * 2 + nB * 32), block(256);
+    if (i >= 1)
+        hipLaunchKernelGGL((k_slab_pair<64, 8, 8, 0, 32, 4, 8, 1, 1>), grid, block, 0, st, repsA, nA, repsB, twA, twB, i);
+    else
+        hipLaunchKernelGGL((k_slab_pair<64, 8, 8, 0, 32, 4, 8, 1, 0>), grid, block, 0, st, repsA, nA, repsB, twA, twB, i);
+    return hipGetLastError();
+}
+
 hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st, int variant) {
     if (!io_aligned(io) || (io.M & 1)) return hipErrorInvalidValue;
     if (io.acc[0] && (n > 64 || mode != CHAIN_PLAIN)) return hipErrorInvalidValue;  // no fused kernel

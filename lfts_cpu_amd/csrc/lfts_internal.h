@@ -215,6 +215,11 @@ hipError_t launch_slab(const Rep* reps, int nact, int n, int i, const double2* t
                        const Rep* one = nullptr, int variant = 0, int fuse = 0);
 // One pass of one replica's chain of kind `mode` (CHAIN_*), over the n planes.
 hipError_t launch_slab_io(int n, int mode, const SlabIO& io, const double2* tw, hipStream_t st, int variant = 0);
+// Pass i of the plain chains of two groups of a set in one launch (edges ea and eb, each in
+// its default factorisation), bit-identical to a launch_slab of each: 64^3 with 32^3.
+bool slab_pair_supported(int ea, int eb);
+hipError_t launch_slab_pair(const Rep* repsA, int nA, int ea, const Rep* repsB, int nB, int eb, int i,
+                            const double2* twA, const double2* twB, hipStream_t st);
 
 // ---- batched element-wise and reduction kernels ----------------------------------------
 // reps: descriptors of the nact active replicas; grid (nb_max, nact).

@@ -843,6 +843,39 @@ __global__ __launch_bounds__((E / 2) * R2 * FPW) void k_slab(const Rep* __restri
     slab_pass<E, R1, R2, CHAIN_PLAIN, TWL, FPW, 0, FIN, false>(io, plane, tw, smem, (int)blockIdx.y);
 }
 
+// Plain chains of two groups of a replica set in one launch: group A (edge EA, one field
+// per workgroup: workgroup (slot * EA + plane) * 2 + field) first, then group B (edge EB,
+// both fields per workgroup: workgroup slot * EB + plane). The two shapes have the same
+// workgroup size, so every thread of a workgroup runs its group's pass and its barriers.
+// Each plane runs the same slab_pass as in k_slab: bit-identical results. A small group
+// whose passes are latency-bound (32^3) then rides in the launches of a large,
+// bandwidth-bound one (64^3) instead of costing a launch of its own per contour step.
+template <int EA, int RA1, int RA2, int TWLA, int EB, int RB1, int RB2, int TWLB, int FIN>
+__global__ __launch_bounds__((EA / 2) * RA2) void k_slab_pair(const Rep* __restrict__ repsA, int nA,
+                                                             const Rep* __restrict__ repsB,
+                                                             const double2* __restrict__ twA,
+                                                             const double2* __restrict__ twB, int i) {
+    using LA = SlabLds<EA, RA1, RA2, TWLA, 1>;
+    using LB = SlabLds<EB, RB1, RB2, TWLB, 2>;
+    static_assert(LA::THREADS == LB::THREADS, "the two shapes must have the same workgroup size");
+    __shared__ __align__(16) unsigned char smem[LA::TOTAL > LB::TOTAL ? LA::TOTAL : LB::TOTAL];
+    const int nwa = nA * EA * 2;
+    if ((int)blockIdx.x < nwa) {
+        const int slot = blockIdx.x / (EA * 2), rest = blockIdx.x - slot * (EA * 2);
+        const Rep& R = repsA[slot];
+        if (i > R.N - 1) return;  // whole workgroup: no barrier is left waiting
+        const SlabIO io = plain_io(R, i);
+        slab_pass<EA, RA1, RA2, CHAIN_PLAIN, TWLA, 1, 0, FIN, false>(io, rest >> 1, twA, smem, rest & 1);
+    } else {
+        const int b = blockIdx.x - nwa;
+        const int slot = b / EB, plane = b - slot * EB;
+        const Rep& R = repsB[slot];
+        if (i > R.N - 1) return;
+        const SlabIO io = plain_io(R, i);
+        slab_pass<EB, RB1, RB2, CHAIN_PLAIN, TWLB, 2, 0, FIN, false>(io, plane, twB, smem, 0);
+    }
+}
+
 // One replica, its pass addresses in the kernel arguments (no dependent descriptor load
 // before the first field loads).
 // One replica's pass has E * 2 / FPW workgroups, at most one per CU: the compiler may

@@ -177,3 +177,33 @@ def test_graph_set_equals_members_alone(gpu, monkeypatch):
         assert np.array_equal(c.get_fields(4), fields[i]), i
         assert c.counters() == counters[i]
         c.close()
+
+
+# ---- a 32^3 group riding in the 64^3 group's launches ----------------------------------------
+
+PAIR_MEMBERS = [
+    # 64^3 and 32^3 members with different chain lengths in each group (the shorter chains
+    # end while the launches go on), and a 48^3 member launched on its own
+    ((64, 64, 64), (4.4, 4.4, 4.4), 14, 6, 31),
+    ((32, 32, 32), (3.6, 3.6, 3.6), 18, 7, 32),
+    ((64, 64, 64), (4.1, 4.1, 4.1), 9, 5, 33),
+    ((32, 32, 32), (3.9, 3.9, 3.9), 11, 4, 34),
+    ((48, 48, 48), (4.0, 4.0, 4.0), 12, 6, 35),
+]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_paired_launch_equals_separate_launches(gpu, monkeypatch, graph):
+    """The 64^3 and 32^3 groups of a set in one launch per contour step (default) give
+    the bits of one launch per group (LFTS_SET_PAIR=0): initial Anderson solve and two
+    Langevin steps, with and without the iteration graphs."""
+    monkeypatch.setenv("LFTS_SET_PAIR", "1")
+    a = _steps(PAIR_MEMBERS, monkeypatch, graph)
+    monkeypatch.setenv("LFTS_SET_PAIR", "0")
+    s = _steps(PAIR_MEMBERS, monkeypatch, graph)
+    assert np.array_equal(a[0], s[0])
+    for (la, ka), (ls, ks) in zip(a[1], s[1]):
+        assert np.array_equal(la, ls) and np.array_equal(ka, ks)
+    for fa, fs in zip(a[2], s[2]):
+        assert np.array_equal(fa, fs)
+    assert a[3] == s[3]
