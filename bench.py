@@ -362,15 +362,23 @@ def driver_leg(w0, m, Lb, N, NA, XbN, C, dt, warmup, steps, lnq_last):
             f.write(f"{m[0]} {m[1]} {m[2]} {Lb[0]!r} {Lb[1]!r} {Lb[2]!r}\n")
             f.write(f"{n_eq} 0 1 {10 * n_eq} 1\n")
             np.savetxt(f, w0, fmt="%.17g")
+        import threading
+
         p = subprocess.Popen([exe, path], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        watchdog = threading.Timer(300.0, p.kill)  # a side leg never holds the bench up
+        watchdog.start()
         tail = []
-        for line in p.stdout:  # the driver ends every line with std::endl (flushed)
-            if line.startswith("lnQ ="):
-                stamps.append(time.perf_counter())
-                lnqs.append(float(line.split("=")[1]))
-            tail = (tail + [line])[-20:]
-        if p.wait() != 0:
-            raise RuntimeError(f"lfts-gpu failed ({p.returncode}): {''.join(tail)}")
+        try:
+            for line in p.stdout:  # the driver ends every line with std::endl (flushed)
+                if line.startswith("lnQ ="):
+                    stamps.append(time.perf_counter())
+                    lnqs.append(float(line.split("=")[1]))
+                tail = (tail + [line])[-20:]
+            rc = p.wait()
+        finally:
+            watchdog.cancel()
+        if rc != 0:
+            raise RuntimeError(f"lfts-gpu failed ({rc}): {''.join(tail)}")
     if len(lnqs) != n_eq:
         raise RuntimeError(f"lfts-gpu printed {len(lnqs)} lnQ lines for {n_eq} steps")
     # the driver prints 6 significant digits: the same trajectory as the timed steps
