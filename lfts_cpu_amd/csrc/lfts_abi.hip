@@ -295,10 +295,32 @@ int upload_coef(lfts_batch* b) {
     return LFTS_OK;
 }
 
-int read_scal(lfts_batch* b) {
-    BHIPCHK(b, hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(double) * lfts::SSTRIDE * b->ctx.size(),
-                              hipMemcpyDeviceToHost, b->st));
+// The reductions store their results in pinned host memory as well as in d_scal, so the
+// host reads them without a copy: one launch fewer per Anderson iteration (one 64^3
+// replica 29.19-29.21 against 29.11-29.15 Langevin steps/s; replica sets within their
+// run-to-run spread). LFTS_SCAL_COPY=1 copies d_scal instead (same values).
+// Summing the Gram row in the phi pass's last workgroup instead of a reduction launch
+// measured slower (28.3 against 28.8 steps/s, sets 161 against 185): each of its
+// workgroups must release its row with a device-scope fence before counting itself in.
+double* host_mirror(const lfts_batch* b) {
+    const char* e = std::getenv("LFTS_SCAL_COPY");
+    return (e && std::strcmp(e, "1") == 0) ? nullptr : b->h_scal_dev;
+}
+
+// one checkpointed context: its calc_concs forms Q in its own pass, into d_scal only
+bool ckpt_single(const lfts_batch* b) { return b->ctx.size() == 1 && b->ctx[0]->ck_stride; }
+
+int read_scal(lfts_batch* b, bool on_host) {
+    if (!on_host || !host_mirror(b))
+        BHIPCHK(b, hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(double) * lfts::SSTRIDE * b->ctx.size(),
+                                  hipMemcpyDeviceToHost, b->st));
     return sync_batch(b);
+}
+
+// the Gram row of every active member into scal and its host mirror
+int reduce_row(lfts_batch* b, int nact, int ncomp) {
+    BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, ncomp, 0, b->d_scal, 0, b->st, host_mirror(b)));
+    return LFTS_OK;
 }
 
 double scal(const lfts_batch* b, int id, int k) { return b->h_scal[(size_t)id * lfts::SSTRIDE + k]; }
@@ -468,7 +490,7 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int 
     const bool whole = part == PART_ALL;
     const int nact = (int)b->act.size();
     if (nact == 0) return LFTS_OK;
-    if (b->ctx.size() == 1 && b->ctx[0]->ck_stride) {  // checkpointed storage: its own phi pass
+    if (ckpt_single(b)) {  // checkpointed storage: its own phi pass
         if (int rc = ckpt_calc_concs(b->ctx[0])) return rc;
         if (dots == DOTS_ONE) return bfail(b, LFTS_EINVAL, "by-value Gram row with checkpointed storage");
         if (dots) BHIPCHK(b, lfts::launch_am_dots(b->d_ract, nact, b->nb_max, b->d_coef, b->d_part, b->st));
@@ -570,7 +592,7 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int 
     BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_qpart, b->st));
     const int nh_one = (dots == DOTS_ONE) ? b->h_coef[b->act[0]].nh_dots : -1;
     BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, b->d_qpart, fuse,
-                                dots == DOTS_COEF ? b->d_coef : nullptr, nh_one, b->d_part, b->st));
+                                dots == DOTS_COEF ? b->d_coef : nullptr, nh_one, b->d_part, b->st, host_mirror(b)));
     if (rec) {
         BHIPCHK(b, hipEventRecord(ev.pe1, b->st));
         ev.npasses = npasses;
@@ -633,9 +655,10 @@ int enqueue_iteration_part(lfts_batch* b, int part) {
         if (int rc = calc_concs_async(b, false, true, DOTS_COEF, PART_CHAIN, &b->it_npasses)) return rc;
     } else {
         if (int rc = calc_concs_async(b, false, true, DOTS_COEF, PART_PHI)) return rc;
-        BHIPCHK(b, lfts::launch_reduce(b->d_ract, n, b->d_part, b->it_ncomp, 0, b->d_scal, 0, b->st));
-        BHIPCHK(b, hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(double) * lfts::SSTRIDE * n, hipMemcpyDeviceToHost,
-                                  b->st));
+        if (int rc = reduce_row(b, n, b->it_ncomp)) return rc;
+        if (!host_mirror(b))
+            BHIPCHK(b, hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(double) * lfts::SSTRIDE * n,
+                                      hipMemcpyDeviceToHost, b->st));
     }
     return LFTS_OK;
 }
@@ -688,7 +711,7 @@ std::vector<int> all_ids(const lfts_batch* b) {
 int batch_calc_concs(lfts_batch* b, double* lnQ) {
     if (int rc = set_active(b, all_ids(b))) return rc;
     if (int rc = calc_concs_async(b, true)) return rc;
-    if (int rc = read_scal(b)) return rc;
+    if (int rc = read_scal(b, !ckpt_single(b))) return rc;
     bool finite = true;
     for (size_t id = 0; id < b->ctx.size(); id++) {
         const double Q = scal(b, (int)id, 0);
@@ -748,8 +771,8 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
             if (!h_ready) {
                 if (int rc = put_flags(b, active, {}, true)) return rc;
                 if (int rc = calc_concs_async(b, false, false, DOTS_COEF)) return rc;
-                BHIPCHK(b, lfts::launch_reduce(b->d_ract, n, b->d_part, b->it_ncomp, 0, b->d_scal, 0, b->st));
-                if (int rc = read_scal(b)) return rc;
+                if (int rc = reduce_row(b, n, b->it_ncomp)) return rc;
+                if (int rc = read_scal(b, true)) return rc;
             } else {
                 if (int rc = put_flags(b, active, upd, false)) return rc;
                 if (int rc = replay_iteration(b)) return rc;
@@ -772,8 +795,8 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
                 const lfts_ctx* c = b->ctx[id];
                 nh_max = std::max(nh_max, (k[id] < c->nhMax + 1) ? k[id] - 1 : c->nhMax);
             }
-            BHIPCHK(b, lfts::launch_reduce(b->d_ract, nact, b->d_part, nh_max + 1, 0, b->d_scal, 0, b->st));
-            if (int rc = read_scal(b)) return rc;
+            if (int rc = reduce_row(b, nact, nh_max + 1)) return rc;
+            if (int rc = read_scal(b, true)) return rc;
         }
         for (int id : active) {
             lfts_ctx* c = b->ctx[id];
@@ -1018,7 +1041,8 @@ int batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
     BCREATE(hipMalloc(&b->d_part, sizeof(double) * lfts::PSTRIDE * lfts::MAX_NB * n));
     BCREATE(hipMalloc(&b->d_scal, sizeof(double) * lfts::SSTRIDE * n));
     BCREATE(hipMalloc(&b->d_qpart, sizeof(double) * lfts::MAX_NB * n));
-    BCREATE(hipHostMalloc(&b->h_scal, sizeof(double) * lfts::SSTRIDE * n, hipHostMallocDefault));
+    BCREATE(hipHostMalloc(&b->h_scal, sizeof(double) * lfts::SSTRIDE * n, hipHostMallocMapped | hipHostMallocCoherent));
+    BCREATE(hipHostGetDevicePointer((void**)&b->h_scal_dev, b->h_scal, 0));
     if (n > 1) {  // flags of the replayed Anderson iteration (all zero: every member runs)
         BCREATE(hipMalloc(&b->d_flags, sizeof(int) * 2 * n));
         BCREATE(hipMemset(b->d_flags, 0, sizeof(int) * 2 * n));
@@ -1340,8 +1364,8 @@ int lfts_hamiltonian(lfts_ctx* c, double XbN, double* H) {
     if (int rc = set_active(b, all_ids(b))) return rc;
     if (int rc = calc_concs_async(b)) return rc;
     HIPCHK(c, lfts::launch_h_sums(b->d_ract, 1, b->nb_max, b->d_part, b->st));
-    HIPCHK(c, lfts::launch_reduce(b->d_ract, 1, b->d_part, 2, 0, b->d_scal, 1, b->st));
-    if (int rc = read_scal(b)) return rc;
+    HIPCHK(c, lfts::launch_reduce(b->d_ract, 1, b->d_part, 2, 0, b->d_scal, 1, b->st, host_mirror(b)));
+    if (int rc = read_scal(b, !ckpt_single(b))) return rc;
     const double lnQ = std::log(scal(b, 0, 0));
     if (H) *H = -lnQ + (scal(b, 0, 2) / XbN - scal(b, 0, 1)) / c->M;
     return std::isfinite(lnQ) ? LFTS_OK : fail(c, LFTS_ENUM, "partition function Q is not finite and positive");

@@ -113,24 +113,17 @@ __device__ __forceinline__ double q_from_partials(const Rep& R, const double* qp
     return res[0] / R.M;
 }
 
-// scal[rid][off + c] = sum over the replica's partials (divided by M when asked), all
-// ncomp (<= PSTRIDE) components together. Thread t adds rows t, t + 256, ... (< nb) in
-// that order. All of its rows are loaded before the first add: the partials were written
-// by workgroups on every XCD, so each load is a trip to memory, and a load-add loop paid
-// one trip per row. Row indexes from nb up are clamped to nb - 1 (in bounds, never added),
-// and every row is PSTRIDE doubles wide, so the loads carry no condition; components from
-// ncomp up are summed but never read.
-__global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, const double* __restrict__ part,
-                                                int ncomp, int div_by_M, double* __restrict__ scal, int off) {
-    constexpr int ROWS = MAX_NB / 256;  // rows per thread (launched with 256 threads)
+// Sum over one replica's partial rows p[0..nb) (PSTRIDE doubles each), all PSTRIDE
+// components together; component c < ncomp lands in res[c], written (and readable) by
+// thread c. Thread t adds rows t, t + 256, ... (< nb) in that order. All of its rows are
+// loaded before the first add: the partials were written by workgroups on every XCD, so
+// each load is a trip to memory, and a load-add loop paid one trip per row. Row indexes
+// from nb up are clamped to nb - 1 (in bounds, never added), and every row is PSTRIDE
+// doubles wide, so the loads carry no condition; components from ncomp up are summed but
+// never stored.
+__device__ __forceinline__ void reduce_rows(const double* p, int nb, int ncomp, double* red, double* res) {
+    constexpr int ROWS = MAX_NB / 256;  // rows per thread (256 threads)
     static_assert(MAX_NB % 256 == 0, "whole rows per thread");
-    __shared__ double red[4 * PSTRIDE];
-    __shared__ double res[PSTRIDE];
-    const int slot = blockIdx.x;
-    const Rep R = reps[slot];
-    if (R.skip && *R.skip) return;
-    const double* p = part + (long)slot * MAX_NB * PSTRIDE;
-    const int nb = R.nb;  // 1 <= nb <= MAX_NB
     double v[ROWS][PSTRIDE];
 #pragma unroll
     for (int k = 0; k < ROWS; k++) {
@@ -148,9 +141,24 @@ __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, co
 #pragma unroll
             for (int c = 0; c < PSTRIDE; c++) s[c] += v[k][c];
     block_sums<PSTRIDE>(s, ncomp, red, res);
+}
+
+// scal[rid][off + c] = sum over the replica's partials (divided by M when asked), c <
+// ncomp (<= PSTRIDE); hscal (pinned host memory, may be null) receives the same values.
+__global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, const double* __restrict__ part,
+                                                int ncomp, int div_by_M, double* __restrict__ scal, int off,
+                                                double* __restrict__ hscal) {
+    __shared__ double red[4 * PSTRIDE];
+    __shared__ double res[PSTRIDE];
+    const int slot = blockIdx.x;
+    const Rep R = reps[slot];
+    if (R.skip && *R.skip) return;
+    reduce_rows(part + (long)slot * MAX_NB * PSTRIDE, R.nb, ncomp, red, res);
     if ((int)threadIdx.x < ncomp) {
         const double v = res[threadIdx.x];
-        scal[(long)R.id * SSTRIDE + off + threadIdx.x] = div_by_M ? v / R.M : v;
+        const long at = (long)R.id * SSTRIDE + off + threadIdx.x;
+        scal[at] = div_by_M ? v / R.M : v;
+        if (hscal) hscal[at] = div_by_M ? v / R.M : v;
     }
 }
 
@@ -256,15 +264,18 @@ __device__ __forceinline__ void phi_row(const Rep& R, double Q, int fuse, double
 // (history depth coef[id].nh_dots) into part, as k_am_dots writes them; nh_one >= 0
 // gives that depth directly (one member, no coefficient array).
 // Q comes from the k_q_sum partials (q_from_partials); workgroup 0 also stores it in
-// scal[id][0] for the host.
+// scal[id][0] and in hscal[id][0] (pinned host memory; hscal may be null).
 __global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, double* __restrict__ scal,
-                                             const double* __restrict__ qpart, int fuse,
+                                             double* __restrict__ hscal, const double* __restrict__ qpart, int fuse,
                                              const Coef* __restrict__ coef, int nh_one, double* __restrict__ part) {
     __shared__ double redn[4 * (MAX_HIST + 1)];
     __shared__ double qred[4], qres[1];
     REPLICA(R);
     const double Q = q_from_partials(R, qpart, qred, qres);
-    if (blockIdx.x == 0 && threadIdx.x == 0) scal[(long)R.id * SSTRIDE] = Q;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[(long)R.id * SSTRIDE] = Q;
+        if (hscal) hscal[(long)R.id * SSTRIDE] = Q;
+    }
     const int nh = nh_one >= 0 ? nh_one : (coef ? coef[R.id].nh_dots : -1);
     if (nh < 0) {
         phi_row<-1>(R, Q, fuse, redn, nullptr);
@@ -542,13 +553,14 @@ hipError_t launch_q_sum(const Rep* reps, int nact, int nb_max, double* qpart, hi
     LAUNCH(k_q_sum, dim3(nb_max, nact), reps, qpart);
 }
 hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncomp, int div_by_M, double* scal,
-                         int scal_off, hipStream_t st) {
-    LAUNCH(k_reduce, dim3(nact), reps, part, ncomp, div_by_M, scal, scal_off);
+                         int scal_off, hipStream_t st, double* hscal) {
+    if (ncomp < 1 || ncomp > PSTRIDE || scal_off < 0 || scal_off + ncomp > SSTRIDE) return hipErrorInvalidValue;
+    LAUNCH(k_reduce, dim3(nact), reps, part, ncomp, div_by_M, scal, scal_off, hscal);
 }
 hipError_t launch_phi(const Rep* reps, int nact, int nb_max, double* scal, const double* qpart, int fuse,
-                      const Coef* coef, int nh_one, double* part, hipStream_t st) {
+                      const Coef* coef, int nh_one, double* part, hipStream_t st, double* hscal) {
     if (nh_one > MAX_HIST || (nh_one >= 0 && nact != 1)) return hipErrorInvalidValue;
-    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal, qpart, fuse, coef, nh_one, part);
+    LAUNCH(k_phi, dim3(nb_max, nact), reps, scal, hscal, qpart, fuse, coef, nh_one, part);
 }
 hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st) {
     LAUNCH(k_am_dots, dim3(nb_max, nact), reps, coef, part);

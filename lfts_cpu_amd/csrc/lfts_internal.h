@@ -227,16 +227,16 @@ hipError_t launch_slab_pair(const Rep* repsA, int nA, int ea, const Rep* repsB, 
 hipError_t launch_prepare_h(const Rep* reps, int nact, int nb_max, hipStream_t st);
 // partial sums of q_N into qpart[slot][workgroup] (MAX_NB doubles per slot)
 hipError_t launch_q_sum(const Rep* reps, int nact, int nb_max, double* qpart, hipStream_t st);
+// hscal (pinned host memory mapped for the device, may be null) receives the same values
 hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncomp, int div_by_M, double* scal,
-                         int scal_off, hipStream_t st);
+                         int scal_off, hipStream_t st, double* hscal = nullptr);
 // phi from the fused sums (replicas with fused_phi(R, fuse)) or from the stored pairs,
-// both in the same order of operations
-// phi from the stored pairs or fused sums, with Q formed from launch_q_sum's partials
-// (also stored in scal[id][0]). coef (may be null): also write each member's next
-// Anderson Gram row partials into part, as launch_am_dots would; nh_one >= 0 (one member
-// only) gives the depth instead of coef.
+// both in the same order of operations, with Q formed from launch_q_sum's partials (also
+// stored in scal[id][0] and hscal[id][0], hscal as for launch_reduce). coef (may be null):
+// also write each member's next Anderson Gram row partials into part, as launch_am_dots
+// would; nh_one >= 0 (one member only) gives the depth instead of coef.
 hipError_t launch_phi(const Rep* reps, int nact, int nb_max, double* scal, const double* qpart, int fuse,
-                      const Coef* coef, int nh_one, double* part, hipStream_t st);
+                      const Coef* coef, int nh_one, double* part, hipStream_t st, double* hscal = nullptr);
 hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st);
 // with_h: also write h = exp(-(+-W- + w+)/N) of the updated fields, as launch_prepare_h would
 hipError_t launch_am_update(const Rep* reps, int nact, int nb_max, const Coef* coef, int with_h, hipStream_t st);

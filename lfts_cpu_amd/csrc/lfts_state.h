@@ -101,7 +101,10 @@ struct lfts_batch {
     double* d_part = nullptr;
     double* d_scal = nullptr;
     double* d_qpart = nullptr;  // k_q_sum partials, MAX_NB per launch slot
-    double* h_scal = nullptr;  // pinned
+    // pinned and coherent: the Gram row and Q reductions also store their results here
+    // (h_scal_dev is its device address), so no copy follows them
+    double* h_scal = nullptr;
+    double* h_scal_dev = nullptr;
     // the active set as launched: slab replicas sorted by edge, then line-path ones
     std::vector<int> act;
     struct Group { int edge, off, cnt; const double2* tw; };
@@ -181,7 +184,10 @@ int calc_concs_async(lfts_batch* b, bool store_all = false, bool h_ready = false
                      int part = PART_ALL, long long* npasses_out = nullptr);
 int slab_variant(lfts_batch* b, int edge, int cnt, const lfts::Rep* d_reps, const lfts::Rep* h_reps,
                  const double2* tw, hipStream_t s, int* variant, int fuse = 0);
-int read_scal(lfts_batch* b);
+// Waits for the set's stream and makes h_scal current: on_host when every value the caller
+// reads was stored there by the kernels (launch_reduce or launch_phi with the host
+// mirror), else with a copy from d_scal first.
+int read_scal(lfts_batch* b, bool on_host = false);
 int sync_batch(lfts_batch* b);
 std::vector<int> all_ids(const lfts_batch* b);
 bool lu_solve(int n, const double* A_in, const double* b, double* x);

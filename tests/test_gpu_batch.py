@@ -207,3 +207,48 @@ def test_paired_launch_equals_separate_launches(gpu, monkeypatch, graph):
     for fa, fs in zip(a[2], s[2]):
         assert np.array_equal(fa, fs)
     assert a[3] == s[3]
+
+
+# ---- reduction results stored in pinned host memory, or copied ------------------------------
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_host_mirrored_scalars_equal_copied_ones_for_a_set(gpu, monkeypatch, graph):
+    """The reductions' results stored by the kernels in pinned host memory (default) give
+    the bits of a copy from device memory (LFTS_SCAL_COPY=1): initial Anderson solve and
+    two Langevin steps of a mixed set."""
+    monkeypatch.setenv("LFTS_SCAL_COPY", "0")
+    a = _steps(PAIR_MEMBERS, monkeypatch, graph)
+    monkeypatch.setenv("LFTS_SCAL_COPY", "1")
+    s = _steps(PAIR_MEMBERS, monkeypatch, graph)
+    assert len(set(a[0].tolist())) > 1
+    assert np.array_equal(a[0], s[0])
+    for (la, ka), (ls, ks) in zip(a[1], s[1]):
+        assert np.array_equal(la, ls) and np.array_equal(ka, ks)
+    for fa, fs in zip(a[2], s[2]):
+        assert np.array_equal(fa, fs)
+    assert a[3] == s[3]
+
+
+def test_host_mirrored_scalars_equal_copied_ones_for_one_replica(gpu, monkeypatch):
+    """One 64^3 replica (fused density sums, Gram row depth as a kernel argument): the
+    same iteration counts, residuals, lnQ, fields and Hamiltonian over an Anderson solve
+    and two Langevin steps whether the scalars are read from host memory or copied."""
+    import torch
+
+    m, Lb, N, NA, seed = (64, 64, 64), (4.4, 4.4, 4.4), 20, 9, 41
+    out = []
+    for copy in ("0", "1"):
+        monkeypatch.setenv("LFTS_SCAL_COPY", copy)
+        (c,), _ = _make([(m, Lb, N, NA, seed)])
+        c.langevin_init(np.random.default_rng(97).normal(0, 1.0, c.M))
+        r = [c.anderson_mix(200, 1e-4)]
+        for s in range(2):
+            e = torch.from_numpy(np.random.default_rng(60 + s).normal(0, 1.0, c.M)).cuda()
+            torch.cuda.synchronize()
+            r.append(c.full_step_dev(e.data_ptr(), 14.0, 0.5))
+        r.append(c.hamiltonian(14.0))
+        out.append((r, c.get_fields(4), c.counters()))
+        c.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
