@@ -589,7 +589,10 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int 
     if (npasses_out) *npasses_out = npasses;
     if (part == PART_CHAIN) return LFTS_OK;
     if (rec) BHIPCHK(b, hipEventRecord(ev.ch1, b->st));
-    BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_qpart, b->st));
+    // Q's partials: the slab chains' last passes leave them (qpl); the others need a pass
+    bool q_pass = false;
+    for (int s = 0; s < nact; s++) q_pass = q_pass || !b->h_ract[s].qpl;
+    if (q_pass) BHIPCHK(b, lfts::launch_q_sum(b->d_ract, nact, b->nb_max, b->d_qpart, b->st));
     const int nh_one = (dots == DOTS_ONE) ? b->h_coef[b->act[0]].nh_dots : -1;
     BHIPCHK(b, lfts::launch_phi(b->d_ract, nact, b->nb_max, b->d_scal, b->d_qpart, fuse,
                                 dots == DOTS_COEF ? b->d_coef : nullptr, nh_one, b->d_part, b->st, host_mirror(b)));
@@ -1041,6 +1044,13 @@ int batch_create(lfts_ctx* const* ctxs, int n, lfts_batch** out) {
     BCREATE(hipMalloc(&b->d_part, sizeof(double) * lfts::PSTRIDE * lfts::MAX_NB * n));
     BCREATE(hipMalloc(&b->d_scal, sizeof(double) * lfts::SSTRIDE * n));
     BCREATE(hipMalloc(&b->d_qpart, sizeof(double) * lfts::MAX_NB * n));
+    // Slab-path chains leave Q's partials (one per plane) themselves, so no pass over q_N
+    // follows them. LFTS_Q_PASS=1 (read when the set is made) keeps that pass instead
+    // (k_q_sum's workgroup partials: another summation order, the same Q to rounding).
+    const char* qp = std::getenv("LFTS_Q_PASS");
+    if (!(qp && std::strcmp(qp, "1") == 0))
+        for (int i = 0; i < n; i++)
+            if (b->reps[i].n && b->reps[i].N >= 2) b->reps[i].qpl = b->d_qpart + (long)i * lfts::MAX_NB;
     BCREATE(hipHostMalloc(&b->h_scal, sizeof(double) * lfts::SSTRIDE * n, hipHostMallocMapped | hipHostMallocCoherent));
     BCREATE(hipHostGetDevicePointer((void**)&b->h_scal_dev, b->h_scal, 0));
     if (n > 1) {  // flags of the replayed Anderson iteration (all zero: every member runs)

@@ -78,26 +78,28 @@ __global__ __launch_bounds__(256) void k_prepare_h(const Rep* __restrict__ reps)
     }
 }
 
-// partial sums of q_N (src/diblock.cc:46-48)
-// into qpart[slot][workgroup]; k_phi forms Q from them
+// partial sums of q_N (src/diblock.cc:46-48) into qpart[id][workgroup], for the members
+// whose chain does not leave its plane sums in qpl (line path); k_phi forms Q from them
 __global__ __launch_bounds__(256) void k_q_sum(const Rep* __restrict__ reps, double* __restrict__ qpart) {
     __shared__ double red[8];
     REPLICA(R);
+    if (R.qpl) return;
     const double* qN = pair_ptr(R, R.N);
     double s = 0.0;
     FOR_POINTS(r, R) s += qN[r];
     s = block_sum(s, red);
-    if (threadIdx.x == 0) qpart[(long)blockIdx.y * MAX_NB + blockIdx.x] = s;
+    if (threadIdx.x == 0) qpart[(long)R.id * MAX_NB + blockIdx.x] = s;
 }
 
-// Q = (sum of the replica's k_q_sum partials) / M, in every workgroup that asks, with
-// k_reduce's operations in k_reduce's order (thread t adds rows t, t + 256, ... < nb, then
-// the wavefront and LDS sums), so every workgroup gets the same bits. All of a thread's
-// rows are loaded first; indexes from nb up are clamped in bounds and never added.
+// Q = (sum of the replica's partials) / M, in every workgroup that asks: the chain's n
+// plane sums (qpl) or k_q_sum's nb workgroup sums. k_reduce's operations in k_reduce's
+// order (thread t adds rows t, t + 256, ... < rows, then the wavefront and LDS sums), so
+// every workgroup gets the same bits. All of a thread's rows are loaded first; indexes
+// from the row count up are clamped in bounds and never added.
 __device__ __forceinline__ double q_from_partials(const Rep& R, const double* qpart, double* red, double* res) {
     constexpr int ROWS = MAX_NB / 256;  // launched with 256 threads
-    const double* p = qpart + (long)blockIdx.y * MAX_NB;
-    const int nb = R.nb;
+    const double* p = R.qpl ? R.qpl : qpart + (long)R.id * MAX_NB;
+    const int nb = R.qpl ? R.n : R.nb;
     double v[ROWS];
 #pragma unroll
     for (int k = 0; k < ROWS; k++) {

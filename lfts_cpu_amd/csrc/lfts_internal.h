@@ -45,6 +45,9 @@ struct Rep {
     int nb;             // workgroups of the element-wise passes, blocks_for(M)
     const int* skip;    // device flag: when set, this replica's kernels return at once (or null)
     double* phacc;      // 4M bead sums of the fused evaluation (slab path), or null
+    // slab path, N >= 2: the chain's last pass stores the sum of q_N over each of its n
+    // planes here (the partials Q is formed from); null: k_q_sum forms nb partials instead
+    double* qpl;
 };
 
 // Per-replica coefficients of one Anderson update or Langevin step.
@@ -123,6 +126,7 @@ struct SlabIO {
     const double* part[2];
     double* acc[2];
     int first[2];
+    double* qsum;          // last pass of a plain chain: field 0's plane sums (q_N, for Q), or null
     long M;
     int finish, cont, xslab;
 };
@@ -166,6 +170,7 @@ __host__ __device__ inline SlabIO plain_io(const Rep& R, int i, int fuse = 0) {
     io.M = M;
     io.finish = i >= 1;
     io.cont = i < R.N - 1;
+    io.qsum = (i >= 1 && i == R.N - 1) ? R.qpl : nullptr;
     io.xslab = (i % 2 == 0);
     const int i0 = (R.N + 1) / 2;
     if (fused_phi(R, fuse) && i >= i0) {
@@ -225,13 +230,15 @@ hipError_t launch_slab_pair(const Rep* repsA, int nA, int ea, const Rep* repsB, 
 // reps: descriptors of the nact active replicas; grid (nb_max, nact).
 // part: [slot][MAX_NB][PSTRIDE]; scal: [replica id][SSTRIDE]
 hipError_t launch_prepare_h(const Rep* reps, int nact, int nb_max, hipStream_t st);
-// partial sums of q_N into qpart[slot][workgroup] (MAX_NB doubles per slot)
+// partial sums of q_N into qpart[id][workgroup] (MAX_NB doubles per member) for the members
+// whose chain does not form them (qpl null)
 hipError_t launch_q_sum(const Rep* reps, int nact, int nb_max, double* qpart, hipStream_t st);
 // hscal (pinned host memory mapped for the device, may be null) receives the same values
 hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncomp, int div_by_M, double* scal,
                          int scal_off, hipStream_t st, double* hscal = nullptr);
 // phi from the fused sums (replicas with fused_phi(R, fuse)) or from the stored pairs,
-// both in the same order of operations, with Q formed from launch_q_sum's partials (also
+// both in the same order of operations, with Q formed from the member's partials (qpl, or
+// launch_q_sum's in qpart) (also
 // stored in scal[id][0] and hscal[id][0], hscal as for launch_reduce). coef (may be null):
 // also write each member's next Anderson Gram row partials into part, as launch_am_dots
 // would; nh_one >= 0 (one member only) gives the depth instead of coef.

@@ -614,6 +614,27 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
     dft<R1, true>(z);
 }
 
+// Sum of one plane of field 0 (the chain's last pass: q_N, whose sum gives Q) into *out,
+// in an order fixed by the field-local thread index t alone, so every kernel and variant
+// (one or two fields per workgroup) gives the same bits: each thread has added its R1
+// values in k order (s), threads t < 32 add every 32nd thread's sum through LDS (red: TPF
+// doubles of the field's own plane buffer), and a width-32 shuffle tree adds theirs.
+// Every thread of the workgroup takes part in the two barriers.
+template <int TPF>
+__device__ __forceinline__ void plane_qsum(double s, double* red, int t, int f, double* out) {
+    static_assert(TPF >= 32, "a field has at least 32 threads");
+    lds_barrier();  // the last readers of the exchange rows are done
+    red[t] = s;
+    lds_barrier();
+    if (f == 0 && t < 32) {  // threads 0..31 of field 0: one wavefront's lanes 0..31
+        double u = red[t];
+        for (int p = t + 32; p < TPF; p += 32) u += red[p];
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) u += __shfl_down(u, o, 32);
+        if (t == 0) *out = u;
+    }
+}
+
 }  // namespace
 
 // LDS of one slab pass shape: FPW exchange/plane buffers (the padded exchange rows of the
@@ -769,6 +790,9 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
         tabB.load(stw, sgb, jb);
     }
 
+    // the chain's last pass adds up its new field-0 values for Q (plane_qsum)
+    const bool qsum = MODE == CHAIN_PLAIN && finish && !cont && io.qsum;
+    double qs = 0.0;
     if constexpr (finish) {
         conv_line<R1, R2>(z, xbuf + c * XP, j, tabA);
         if constexpr (!EARLY) load_finish();
@@ -792,13 +816,20 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
                 z[k].y = fma(az[k].y, sz[k].y, z[k].y);
             }
             if (!PLAIN || qout) st2_nt(qout + p, z[k]);
+            if (qsum) {
+                qs = (k == 0) ? z[k].x : qs + z[k].x;
+                qs += z[k].y;
+            }
             if constexpr (MODE == CHAIN_ADJOINT) {
                 z[k].x *= hz[k].x;
                 z[k].y *= hz[k].y;
             }
         }
     }
-    if (!cont) return;
+    if (!cont) {
+        if (qsum) plane_qsum<TPF>(qs, rp, t, f, io.qsum + plane);
+        return;
+    }
 
     conv_line<R1, R2>(z, xbuf + c * XP, j, tabA);
     // re-pair the real plane: rows (r, r + H) become the complex z-lines
