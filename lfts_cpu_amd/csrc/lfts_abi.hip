@@ -647,7 +647,8 @@ int clear_flags(lfts_batch* b) {
 // One Anderson iteration after the first, every member in the launches, in three parts:
 // (0) flags and coefficients (pinned staging, read when the graph runs) and the update
 // with the Boltzmann factors; (1) the contour-step chains on the groups' streams; (2) Q,
-// phi with the Gram row, its reduction and read-back.
+// phi with the Gram row and its reduction (into pinned host memory; a copy with
+// LFTS_SCAL_COPY=1).
 int enqueue_iteration_part(lfts_batch* b, int part) {
     const int n = (int)b->ctx.size();
     if (part == 0) {

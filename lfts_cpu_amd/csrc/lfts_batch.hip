@@ -265,7 +265,7 @@ __device__ __forceinline__ void phi_row(const Rep& R, double Q, int fuse, double
 // With coef, also the Gram row partials of each member's next Anderson iteration
 // (history depth coef[id].nh_dots) into part, as k_am_dots writes them; nh_one >= 0
 // gives that depth directly (one member, no coefficient array).
-// Q comes from the k_q_sum partials (q_from_partials); workgroup 0 also stores it in
+// Q comes from the member's partials (q_from_partials); workgroup 0 also stores it in
 // scal[id][0] and in hscal[id][0] (pinned host memory; hscal may be null).
 __global__ __launch_bounds__(256) void k_phi(const Rep* __restrict__ reps, double* __restrict__ scal,
                                              double* __restrict__ hscal, const double* __restrict__ qpart, int fuse,

@@ -238,10 +238,10 @@ hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncom
                          int scal_off, hipStream_t st, double* hscal = nullptr);
 // phi from the fused sums (replicas with fused_phi(R, fuse)) or from the stored pairs,
 // both in the same order of operations, with Q formed from the member's partials (qpl, or
-// launch_q_sum's in qpart) (also
-// stored in scal[id][0] and hscal[id][0], hscal as for launch_reduce). coef (may be null):
-// also write each member's next Anderson Gram row partials into part, as launch_am_dots
-// would; nh_one >= 0 (one member only) gives the depth instead of coef.
+// launch_q_sum's in qpart) and stored in scal[id][0] and hscal[id][0] (hscal as for
+// launch_reduce). coef (may be null): also write each member's next Anderson Gram row
+// partials into part, as launch_am_dots would; nh_one >= 0 (one member only) gives the
+// depth instead of coef.
 hipError_t launch_phi(const Rep* reps, int nact, int nb_max, double* scal, const double* qpart, int fuse,
                       const Coef* coef, int nh_one, double* part, hipStream_t st, double* hscal = nullptr);
 hipError_t launch_am_dots(const Rep* reps, int nact, int nb_max, const Coef* coef, double* part, hipStream_t st);

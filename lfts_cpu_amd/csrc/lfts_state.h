@@ -100,7 +100,7 @@ struct lfts_batch {
     lfts::Coef* h_coef = nullptr;  // pinned
     double* d_part = nullptr;
     double* d_scal = nullptr;
-    double* d_qpart = nullptr;  // k_q_sum partials, MAX_NB per launch slot
+    double* d_qpart = nullptr;  // Q's partials, MAX_NB per member: plane sums (slab) or k_q_sum's
     // pinned and coherent: the Gram row and Q reductions also store their results here
     // (h_scal_dev is its device address), so no copy follows them
     double* h_scal = nullptr;
@@ -129,7 +129,7 @@ struct lfts_batch {
     long long prof_n_pass = 0, prof_n_pe = 0;
     // A set of two or more slab-path members runs each Anderson iteration after the first
     // as one HIP graph (it_exec): coefficient and flag uploads, the update, the propagator
-    // evaluation on the groups' streams, the Gram row and its read-back. Every member stays
+    // evaluation on the groups' streams, the Gram row and its reduction. Every member stays
     // in every launch of the graph; per-member flags (d_flags: n evaluation flags, then n
     // update flags, each member's descriptors pointing at its own) make the kernels of a
     // member that has left the loop return at once.
@@ -137,7 +137,7 @@ struct lfts_batch {
     int* h_flags = nullptr;       // pinned staging
     lfts::Rep* d_rupd = nullptr;  // all members' descriptors with the update flags (launch order)
     // three graphs: before the chains (uploads, update), the chains, after them (Q, phi and
-    // the Gram row, read-back), so that live timing can record events between them
+    // the Gram row), so that live timing can record events between them
     hipGraph_t it_graph[3] = {};
     hipGraphExec_t it_exec[3] = {};
     long long it_npasses = 0;     // contour-step launches of the chain graph
