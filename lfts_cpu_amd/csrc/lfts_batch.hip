@@ -147,9 +147,11 @@ __device__ __forceinline__ void reduce_rows(const double* p, int nb, int ncomp, 
 
 // scal[rid][off + c] = sum over the replica's partials (divided by M when asked), c <
 // ncomp (<= PSTRIDE); hscal (pinned host memory, may be null) receives the same values.
+// hseq (pinned host memory, may be null): once the results are stored, hseq[slot] = seq,
+// so that the host can wait for them by polling instead of synchronising the stream.
 __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, const double* __restrict__ part,
                                                 int ncomp, int div_by_M, double* __restrict__ scal, int off,
-                                                double* __restrict__ hscal) {
+                                                double* __restrict__ hscal, unsigned* __restrict__ hseq, unsigned seq) {
     __shared__ double red[4 * PSTRIDE];
     __shared__ double res[PSTRIDE];
     const int slot = blockIdx.x;
@@ -161,6 +163,10 @@ __global__ __launch_bounds__(256) void k_reduce(const Rep* __restrict__ reps, co
         const long at = (long)R.id * SSTRIDE + off + threadIdx.x;
         scal[at] = div_by_M ? v / R.M : v;
         if (hscal) hscal[at] = div_by_M ? v / R.M : v;
+    }
+    if (hseq && threadIdx.x < 64) {  // wave 0 stored the results (ncomp <= 16)
+        __threadfence_system();
+        if (threadIdx.x == 0) __hip_atomic_store(hseq + slot, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -555,9 +561,9 @@ hipError_t launch_q_sum(const Rep* reps, int nact, int nb_max, double* qpart, hi
     LAUNCH(k_q_sum, dim3(nb_max, nact), reps, qpart);
 }
 hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncomp, int div_by_M, double* scal,
-                         int scal_off, hipStream_t st, double* hscal) {
+                         int scal_off, hipStream_t st, double* hscal, unsigned* hseq, unsigned seq) {
     if (ncomp < 1 || ncomp > PSTRIDE || scal_off < 0 || scal_off + ncomp > SSTRIDE) return hipErrorInvalidValue;
-    LAUNCH(k_reduce, dim3(nact), reps, part, ncomp, div_by_M, scal, scal_off, hscal);
+    LAUNCH(k_reduce, dim3(nact), reps, part, ncomp, div_by_M, scal, scal_off, hscal, hseq, seq);
 }
 hipError_t launch_phi(const Rep* reps, int nact, int nb_max, double* scal, const double* qpart, int fuse,
                       const Coef* coef, int nh_one, double* part, hipStream_t st, double* hscal) {

@@ -233,9 +233,11 @@ hipError_t launch_prepare_h(const Rep* reps, int nact, int nb_max, hipStream_t s
 // partial sums of q_N into qpart[id][workgroup] (MAX_NB doubles per member) for the members
 // whose chain does not form them (qpl null)
 hipError_t launch_q_sum(const Rep* reps, int nact, int nb_max, double* qpart, hipStream_t st);
-// hscal (pinned host memory mapped for the device, may be null) receives the same values
+// hscal (pinned host memory mapped for the device, may be null) receives the same values;
+// hseq (pinned, may be null): hseq[slot] = seq once a slot's results are stored
 hipError_t launch_reduce(const Rep* reps, int nact, const double* part, int ncomp, int div_by_M, double* scal,
-                         int scal_off, hipStream_t st, double* hscal = nullptr);
+                         int scal_off, hipStream_t st, double* hscal = nullptr, unsigned* hseq = nullptr,
+                         unsigned seq = 0);
 // phi from the fused sums (replicas with fused_phi(R, fuse)) or from the stored pairs,
 // both in the same order of operations, with Q formed from the member's partials (qpl, or
 // launch_q_sum's in qpart) and stored in scal[id][0] and hscal[id][0] (hscal as for

@@ -105,6 +105,11 @@ struct lfts_batch {
     // (h_scal_dev is its device address), so no copy follows them
     double* h_scal = nullptr;
     double* h_scal_dev = nullptr;
+    // pinned and coherent: per launch slot, the sequence number of the last reduction whose
+    // results are stored (the host polls it instead of synchronising the stream)
+    unsigned* h_seq = nullptr;
+    unsigned* h_seq_dev = nullptr;
+    unsigned seq = 0;  // wraps around
     // the active set as launched: slab replicas sorted by edge, then line-path ones
     std::vector<int> act;
     struct Group { int edge, off, cnt; const double2* tw; };

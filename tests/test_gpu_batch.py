@@ -232,13 +232,16 @@ def test_host_mirrored_scalars_equal_copied_ones_for_a_set(gpu, monkeypatch, gra
 def test_host_mirrored_scalars_equal_copied_ones_for_one_replica(gpu, monkeypatch):
     """One 64^3 replica (fused density sums, Gram row depth as a kernel argument): the
     same iteration counts, residuals, lnQ, fields and Hamiltonian over an Anderson solve
-    and two Langevin steps whether the scalars are read from host memory or copied."""
+    and two Langevin steps whether the scalars are read from host memory as the host
+    polls for them (default), read from host memory after a stream synchronisation
+    (LFTS_SPIN=0), or copied (LFTS_SCAL_COPY=1)."""
     import torch
 
     m, Lb, N, NA, seed = (64, 64, 64), (4.4, 4.4, 4.4), 20, 9, 41
     out = []
-    for copy in ("0", "1"):
+    for copy, spin in (("0", "1"), ("0", "0"), ("1", "1")):
         monkeypatch.setenv("LFTS_SCAL_COPY", copy)
+        monkeypatch.setenv("LFTS_SPIN", spin)
         (c,), _ = _make([(m, Lb, N, NA, seed)])
         c.langevin_init(np.random.default_rng(97).normal(0, 1.0, c.M))
         r = [c.anderson_mix(200, 1e-4)]
@@ -249,6 +252,7 @@ def test_host_mirrored_scalars_equal_copied_ones_for_one_replica(gpu, monkeypatc
         r.append(c.hamiltonian(14.0))
         out.append((r, c.get_fields(4), c.counters()))
         c.close()
-    assert out[0][0] == out[1][0]
-    assert np.array_equal(out[0][1], out[1][1])
-    assert out[0][2] == out[1][2]
+    for r, f, n in out[1:]:
+        assert r == out[0][0]
+        assert np.array_equal(f, out[0][1])
+        assert n == out[0][2]
