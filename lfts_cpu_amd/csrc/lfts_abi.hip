@@ -158,10 +158,11 @@ double* pair_ptr(lfts_ctx* c, int i) { return (i == 1) ? c->h : c->q + 2L * (i -
 
 // Line path, src/step.cc:35-65: out = h_sel * conv_g(in) for the pair (q_i, q+_{N+1-i}),
 // as three launches (z, y, then x with the Boltzmann factor); `tmp` holds 2M doubles.
+// skip: the device flag that makes the launches return at once (the replica's flag in the
+// set that runs it), or null.
 int line_contour_step(lfts_ctx* c, const double* in, double* out, const double* h, int i, double* tmp,
-                      hipStream_t st) {
+                      hipStream_t st, const int* skip) {
     const long M = c->M;
-    const int* skip = c->self ? c->self->reps[0].skip : nullptr;
     LineArgs z = line_args(c, 2, lfts::LINE_CONV);
     z.inA = in; z.inB = in + M; z.outA = tmp; z.outB = tmp + M;
     z.skip = skip;
@@ -610,7 +611,8 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int 
     for (int id : b->line_reps) {
         lfts_ctx* c = b->ctx[id];
         for (int i = 1; i < c->N; i++) {
-            if (int rc = line_contour_step(c, pair_ptr(c, i), pair_ptr(c, i + 1), c->h, i, c->T0, b->st))
+            if (int rc = line_contour_step(c, pair_ptr(c, i), pair_ptr(c, i + 1), c->h, i, c->T0, b->st,
+                                           b->reps[id].skip))
                 return bfail(b, rc, c->err);
             npasses++;
         }
@@ -650,12 +652,13 @@ int calc_concs_async(lfts_batch* b, bool store_all, bool h_ready, int dots, int 
 
 // ---- the Anderson iteration of a replica set as one HIP graph ---------------------------
 
-// Graph mode: sets of two or more members, all on the slab path (the line-path chain takes
-// its flag from the member's own context), unless LFTS_GRAPH=0.
+// Graph mode: sets of two or more members with every pair stored (the checkpointed storage
+// runs its own chains), unless LFTS_GRAPH=0. Slab-path and line-path members alike take
+// their evaluation flags from the set's descriptors.
 bool graph_mode(const lfts_batch* b) {
     if (b->ctx.size() < 2 || !b->d_flags) return false;
     for (const lfts_ctx* c : b->ctx)
-        if (!c->fused || c->ck_stride) return false;
+        if (c->ck_stride) return false;
     const char* ge = std::getenv("LFTS_GRAPH");
     return !(ge && std::strcmp(ge, "0") == 0);
 }
@@ -1443,7 +1446,9 @@ int lfts_step_fwd(lfts_ctx* c, const double* q_in, double* q_out, const double* 
     if (!c->test) HIPCHK(c, hipMalloc(&c->test, sizeof(double) * 8 * M));
     HIPCHK(c, hipMemcpyAsync(c->test, q_in, sizeof(double) * 2 * M, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(c->test + 2 * M, h, sizeof(double) * 2 * M, hipMemcpyHostToDevice, st));
-    if (int rc = line_contour_step(c, c->test, c->test + 4 * M, c->test + 2 * M, i, c->test + 6 * M, st)) return rc;
+    if (int rc = line_contour_step(c, c->test, c->test + 4 * M, c->test + 2 * M, i, c->test + 6 * M, st,
+                                   c->self ? c->self->reps[0].skip : nullptr))
+        return rc;
     HIPCHK(c, hipMemcpyAsync(q_out, c->test + 4 * M, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     return LFTS_OK;

@@ -171,7 +171,7 @@ int bfail(lfts_batch* b, int code, const std::string& msg);
 lfts::LineArgs line_args(const lfts_ctx* c, int axis, int mode);
 double* pair_ptr(lfts_ctx* c, int i);
 int line_contour_step(lfts_ctx* c, const double* in, double* out, const double* h, int i, double* tmp,
-                      hipStream_t st);
+                      hipStream_t st, const int* skip);
 int set_active(lfts_batch* b, const std::vector<int>& ids);
 // store_all = false lets slab-path replicas fuse the density sums into the chain, which
 // keeps only pairs 1..ceil(N/2) and N; store_all = true keeps every pair.

@@ -160,6 +160,21 @@ def test_graph_iterations_equal_direct_launches(gpu, monkeypatch):
     assert ms_chain > 0 and ms_pe >= ms_chain and n_pass > 0 and n_pe == d[4][3]
 
 
+def test_graph_iterations_with_line_path_members(gpu, monkeypatch):
+    """A set with a line-path member (12 x 10 x 8 mesh) beside slab-path members replays
+    its iterations as graphs too: the same bits, iteration counts and counters as the
+    direct launches."""
+    g = _steps(MEMBERS, monkeypatch, True)
+    d = _steps(MEMBERS, monkeypatch, False)
+    assert np.array_equal(g[0], d[0])
+    for (lg, kg), (ld, kd) in zip(g[1], d[1]):
+        assert np.array_equal(lg, ld) and np.array_equal(kg, kd)
+    for fg, fd in zip(g[2], d[2]):
+        assert np.array_equal(fg, fd)
+    assert g[3] == d[3]
+    assert g[4][3] == d[4][3]  # evaluations counted alike
+
+
 def test_graph_set_equals_members_alone(gpu, monkeypatch):
     import torch
 
