@@ -645,9 +645,16 @@ template <int E, int R1, int R2, int TWL, int FPW>
 struct SlabLds {
     static constexpr int H = E / 2;                      // complex lines per plane
     static constexpr int XP = (R1 * (R2 + 1)) | 1;       // exchange row pitch (complex): padded line, odd
+    // z-line exchange rows (8 threads per line, edges up to 64): a z-line's threads are
+    // consecutive lanes, so the pitch is the first one from the line's extent that is 8 mod
+    // 16 instead (neighbouring lines then start 32 banks apart; an odd pitch put them on
+    // the same banks): 40 for 24 and 32 points, 56 for 40 and 48, 72 for 56 and 64, each
+    // within the plane buffer's size (RBYTES).
+    static constexpr int ZEXT = (R1 - 1) * (R2 + 1) + R2;  // a line's rows end below this
+    static constexpr int XPZ = (R2 == 8 && E <= 64) ? ZEXT + ((24 - ZEXT % 16) % 16) : XP;
     static constexpr int RP = E + ((E + 8) % 16 == 8 ? 8 : 16);  // real plane row pitch (doubles), = 8 mod
                                                                  // 16: rows r, r+1 on other banks
-    static constexpr int XBYTES = H * XP * 16;
+    static constexpr int XBYTES = H * (XP > XPZ ? XP : XPZ) * 16;
     static constexpr int RBYTES = E * RP * 8;
     static constexpr int BYTES = XBYTES > RBYTES ? XBYTES : RBYTES;
     static constexpr int TBYTES = TWL ? E * 16 + 2 * E * 8 : 0;
@@ -688,6 +695,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     constexpr int H = Lds::H;
     constexpr int TPF = H * R2;        // threads per field
     constexpr int XP = Lds::XP;
+    constexpr int XPZ = Lds::XPZ;
     constexpr int RP = Lds::RP;
     constexpr int BYTES = Lds::BYTES;
     constexpr bool EXTRA = MODE != CHAIN_PLAIN;
@@ -846,7 +854,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
         const int b = jb + R2 * k;
         w[k] = make_double2(rp[r * RP + b], rp[(r + H) * RP + b]);
     }
-    conv_line<R1, R2>(w, xbuf + r * XP, jb, tabB);
+    conv_line<R1, R2>(w, xbuf + r * XPZ, jb, tabB);
     double* Tout = io.tout;
 #pragma unroll
     for (int k = 0; k < R1; k++) {
