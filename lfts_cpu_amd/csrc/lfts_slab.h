@@ -645,13 +645,14 @@ template <int E, int R1, int R2, int TWL, int FPW>
 struct SlabLds {
     static constexpr int H = E / 2;                      // complex lines per plane
     static constexpr int XP = (R1 * (R2 + 1)) | 1;       // exchange row pitch (complex): padded line, odd
-    // z-line exchange rows (8 threads per line, edges up to 64): a z-line's threads are
-    // consecutive lanes, so the pitch is the first one from the line's extent that is 8 mod
-    // 16 instead (neighbouring lines then start 32 banks apart; an odd pitch put them on
-    // the same banks): 40 for 24 and 32 points, 56 for 40 and 48, 72 for 56 and 64, each
-    // within the plane buffer's size (RBYTES).
+    // z-line exchange rows (8 threads per line): a z-line's threads are consecutive lanes,
+    // so the pitch is the first one from the line's extent that is 8 mod 16 instead
+    // (neighbouring lines then start 32 banks apart; an odd pitch put them on the same
+    // banks): 40 for 24 and 32 points, 56 for 40 and 48, 72 for 56 and 64, within the plane
+    // buffer's size; 120, 136, 136 and 152 for 96, 112, 120 and 128 points, which grows the
+    // largest workgroup's LDS to 159744 bytes (128 points, tables included).
     static constexpr int ZEXT = (R1 - 1) * (R2 + 1) + R2;  // a line's rows end below this
-    static constexpr int XPZ = (R2 == 8 && E <= 64) ? ZEXT + ((24 - ZEXT % 16) % 16) : XP;
+    static constexpr int XPZ = (R2 == 8) ? ZEXT + ((24 - ZEXT % 16) % 16) : XP;
     static constexpr int RP = E + ((E + 8) % 16 == 8 ? 8 : 16);  // real plane row pitch (doubles), = 8 mod
                                                                  // 16: rows r, r+1 on other banks
     static constexpr int XBYTES = H * (XP > XPZ ? XP : XPZ) * 16;
