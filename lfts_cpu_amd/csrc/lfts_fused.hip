@@ -161,7 +161,9 @@ __device__ __forceinline__ long axis_line_base(const LineArgs& a, long line) {
 template <int E, int R1, int R2, bool ZAX>
 __global__ __launch_bounds__(AX_LINES * R2) void k_axis(const LineArgs a) {
     static_assert(R1 * R2 == E, "unsupported factorisation");
-    constexpr int XP = (R1 * (R2 + 1)) | 1;  // exchange row pitch (complex), odd
+    // exchange row pitch (complex): odd, or for z-lines (a line's threads adjacent) the
+    // conflict-free zline_pitch
+    constexpr int XP = ZAX ? zline_pitch(R1, R2, (R1 * (R2 + 1)) | 1) : (R1 * (R2 + 1)) | 1;
     constexpr int XBYTES = AX_LINES * XP * 16;
     __shared__ __align__(16) unsigned char smem[XBYTES + E * 16 + E * 8];
     if (a.skip && *a.skip) return;

@@ -637,6 +637,14 @@ __device__ __forceinline__ void plane_qsum(double s, double* red, int t, int f, 
 
 }  // namespace
 
+// Row pitch (complex entries) of the exchange rows of lines whose R2 = 8 threads are
+// consecutive lanes: the first pitch from the line's extent, (R1 - 1)(R2 + 1) + R2, that is
+// 8 mod 16, so neighbouring lines start 32 banks apart (an odd pitch put them on the same
+// banks); other shapes keep the odd pitch xp.
+constexpr int zline_pitch(int r1, int r2, int xp) {
+    return r2 == 8 ? ((r1 - 1) * (r2 + 1) + r2) + ((24 - ((r1 - 1) * (r2 + 1) + r2) % 16) % 16) : xp;
+}
+
 // LDS of one slab pass shape: FPW exchange/plane buffers (the padded exchange rows of the
 // line DFTs, reused for the re-paired real plane) and, with TWL = 1, the tables. The
 // kernel owns the array (a kernel that runs several shapes sizes it for the largest) and
@@ -645,14 +653,11 @@ template <int E, int R1, int R2, int TWL, int FPW>
 struct SlabLds {
     static constexpr int H = E / 2;                      // complex lines per plane
     static constexpr int XP = (R1 * (R2 + 1)) | 1;       // exchange row pitch (complex): padded line, odd
-    // z-line exchange rows (8 threads per line): a z-line's threads are consecutive lanes,
-    // so the pitch is the first one from the line's extent that is 8 mod 16 instead
-    // (neighbouring lines then start 32 banks apart; an odd pitch put them on the same
-    // banks): 40 for 24 and 32 points, 56 for 40 and 48, 72 for 56 and 64, within the plane
-    // buffer's size; 120, 136, 136 and 152 for 96, 112, 120 and 128 points, which grows the
-    // largest workgroup's LDS to 159744 bytes (128 points, tables included).
-    static constexpr int ZEXT = (R1 - 1) * (R2 + 1) + R2;  // a line's rows end below this
-    static constexpr int XPZ = (R2 == 8) ? ZEXT + ((24 - ZEXT % 16) % 16) : XP;
+    // z-line exchange rows (zline_pitch): 40 for 24 and 32 points, 56 for 40 and 48, 72 for
+    // 56 and 64, within the plane buffer's size; 120, 136, 136 and 152 for 96, 112, 120 and
+    // 128 points, which grows the largest workgroup's LDS to 159744 bytes (128 points,
+    // tables included).
+    static constexpr int XPZ = zline_pitch(R1, R2, XP);
     static constexpr int RP = E + ((E + 8) % 16 == 8 ? 8 : 16);  // real plane row pitch (doubles), = 8 mod
                                                                  // 16: rows r, r+1 on other banks
     static constexpr int XBYTES = H * (XP > XPZ ? XP : XPZ) * 16;
