@@ -921,8 +921,8 @@ int batch_anderson_mix(lfts_batch* b, int maxIter, double errTol, int* k_ret, do
 // ---- Anderson mixing on the device, one context -----------------------------------------
 
 // The device-resident loop is chosen with LFTS_ANDERSON=device. It is not the default:
-// at 64^3, N=90 the host loop (one read-back and a host LU per iteration) measured 29.4
-// Langevin steps/s against 28.1 for the device loop (its two extra kernels per
+// at 64^3, N=90 the host loop (one read-back and a host LU per iteration) measured 29.6
+// Langevin steps/s against 28.3 for the device loop (its two extra kernels per
 // iteration, the flag test in every kernel and the speculative tail cost more than the
 // round trip saves); graph replay of an iteration made no measurable difference.
 bool device_anderson_enabled(const lfts_ctx* c) {
