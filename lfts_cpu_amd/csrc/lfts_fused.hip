@@ -189,7 +189,7 @@ __global__ __launch_bounds__(AX_LINES * R2) void k_axis(const LineArgs a) {
     lds_barrier();
     LdsTab<R1, R2> tab;
     tab.load(stw, sg, j);
-    conv_line<R1, R2>(z, reinterpret_cast<double2*>(smem) + c * XP, j, tab);
+    conv_line<R1, R2, false>(z, reinterpret_cast<double2*>(smem) + c * XP, j, tab);  // first use
     if (!live) return;
 #pragma unroll
     for (int k = 0; k < R1; k++) {

@@ -573,14 +573,19 @@ struct LdsTab {
 // 1D circular convolution of one complex line of length N = R1*R2 held by R2 threads
 // (this thread: index j, elements z[k] = x[j + R2 k]). lx is the line's LDS exchange row
 // (R1*R2 complex); tab supplies the thread's twiddles exp(-2 pi i jk/N) and kernel values.
-template <int R1, int R2, class Tab>
+// The thread writes its column (entries k Q + j), reads back rows, rewrites them and reads
+// its column again. SYNC_IN: wait for the workgroup before the first write, because the
+// exchange rows were last used otherwise; without it the last LDS accesses to them must
+// have been the same threads' reads of their own columns (a convolution of the same lines)
+// or none.
+template <int R1, int R2, bool SYNC_IN, class Tab>
 __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const Tab& tab) {
     constexpr int U = stage2_per_thread<R1, R2>();
     constexpr int Q = R2 + 1;  // padded run length in LDS
     dft<R1, false>(z);
 #pragma unroll
     for (int k = 1; k < R1; k++) z[k] = c_mul(z[k], tab.w(k));
-    lds_barrier();  // previous readers of the exchange rows are done
+    if constexpr (SYNC_IN) lds_barrier();  // previous readers of the exchange rows are done
 #pragma unroll
     for (int k = 0; k < R1; k++) lx[k * Q + j] = z[k];
     lds_barrier();
@@ -598,7 +603,9 @@ __device__ __forceinline__ void conv_line(double2* z, double2* lx, int j, const 
             inv_dft_g<R2>(y + u * R2, gl);
         }
     }
-    lds_barrier();
+    // no barrier: each row d is read and rewritten by one thread only (d = stage2_dft(j, u)
+    // takes every row once over the line's threads), and a wavefront's LDS accesses are
+    // performed in order
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const int d = stage2_dft<R1, R2>(j, u);
@@ -808,7 +815,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
     const bool qsum = MODE == CHAIN_PLAIN && finish && !cont && io.qsum;
     double qs = 0.0;
     if constexpr (finish) {
-        conv_line<R1, R2>(z, xbuf + c * XP, j, tabA);
+        conv_line<R1, R2, false>(z, xbuf + c * XP, j, tabA);  // the exchange rows' first use
         if constexpr (!EARLY) load_finish();
         double* qout = io.qout;
 #pragma unroll
@@ -845,7 +852,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
         return;
     }
 
-    conv_line<R1, R2>(z, xbuf + c * XP, j, tabA);
+    conv_line<R1, R2, false>(z, xbuf + c * XP, j, tabA);  // first use, or the same lines again
     // re-pair the real plane: rows (r, r + H) become the complex z-lines
     lds_barrier();
 #pragma unroll
@@ -860,7 +867,7 @@ __device__ __forceinline__ void slab_pass(const SlabIO& io, int plane, const dou
         const int b = jb + R2 * k;
         w[k] = make_double2(rp[r * RP + b], rp[(r + H) * RP + b]);
     }
-    conv_line<R1, R2>(w, xbuf + r * XPZ, jb, tabB);
+    conv_line<R1, R2, true>(w, xbuf + r * XPZ, jb, tabB);  // after the re-pairing reads
     double* Tout = io.tout;
 #pragma unroll
     for (int k = 0; k < R1; k++) {
