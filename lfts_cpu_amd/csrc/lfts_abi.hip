@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -340,12 +341,15 @@ bool spin_on() {
 
 // Waits until slots 0..nact-1 carry the last sequence number. Every 1024 polls the stream
 // is queried: once it has drained the flags must be set (an error otherwise), and a
-// stream error ends the wait.
+// stream error ends the wait. The reduction stores its results, fences at system scope,
+// then stores the flag; the acquire fence after the flags are seen keeps the compiler
+// from reading the results earlier (x86 keeps loads in order by itself).
 int wait_row(lfts_batch* b, int nact) {
     const volatile unsigned* f = b->h_seq;
     auto done = [&]() {
         for (int s = 0; s < nact; s++)
             if (f[s] != b->seq) return false;
+        std::atomic_thread_fence(std::memory_order_acquire);
         return true;
     };
     for (long it = 1;; it++) {
