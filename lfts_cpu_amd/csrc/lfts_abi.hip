@@ -54,7 +54,7 @@ bool factorize(int n, std::vector<int>& r) {
     r.clear();
     while (n % 4 == 0) { r.push_back(4); n /= 4; }
     while (n % 2 == 0) { r.push_back(2); n /= 2; }
-    for (int p : {3, 5, 7})
+    for (int p : {3, 5, 7, 11, 13})
         while (n % p == 0) { r.push_back(p); n /= p; }
     return n == 1;
 }
@@ -1127,7 +1127,7 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     if (desc->nhMax < 1 || desc->nhMax > lfts::MAX_HIST) return LFTS_EINVAL;
     for (int d = 0; d < 3; d++) {
         if (desc->m[d] < 1 || !(desc->L[d] > 0.0)) return LFTS_EINVAL;
-        std::vector<int> rad;  // the line transforms take factors 2, 3, 5 and 7 only
+        std::vector<int> rad;  // the line transforms take factors 2, 3, 5, 7, 11 and 13 only
         if (!factorize(desc->m[d], rad) || (int)rad.size() > lfts::MAXST) return LFTS_EINVAL;
     }
     int ndev = 0;

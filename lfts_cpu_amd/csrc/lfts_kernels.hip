@@ -107,7 +107,9 @@ __device__ void line_stage(double2* lds, int pitch, int T, int n, int B, const d
     }
 }
 
-template <bool INV>
+// BIG: also the radix-11 and radix-13 stages (direct DFTs), in a kernel of their own, since
+// their registers would halve the occupancy of the kernel that serves the other lengths
+template <bool INV, bool BIG>
 __device__ void run_stage(int r, double2* lds, int pitch, int T, int n, int B, const double2* tw) {
     switch (r) {
         case 2: line_stage<2, INV>(lds, pitch, T, n, B, tw); break;
@@ -115,6 +117,12 @@ __device__ void run_stage(int r, double2* lds, int pitch, int T, int n, int B, c
         case 4: line_stage<4, INV>(lds, pitch, T, n, B, tw); break;
         case 5: line_stage<5, INV>(lds, pitch, T, n, B, tw); break;
         case 7: line_stage<7, INV>(lds, pitch, T, n, B, tw); break;
+        case 11:
+            if constexpr (BIG) line_stage<11, INV>(lds, pitch, T, n, B, tw);
+            break;
+        case 13:
+            if constexpr (BIG) line_stage<13, INV>(lds, pitch, T, n, B, tw);
+            break;
         default: break;  // plans only contain the radices above (checked on the host)
     }
 }
@@ -136,6 +144,7 @@ __device__ __forceinline__ long line_base(const LineArgs& a, long line) {
 //  mode LINE_IFFT: natural-order complex in -> inverse FFT -> complex out, or its real
 //                  part (outR).
 //  mode LINE_CMUL: complex in -> FFT -> *mul (per point) -> IFFT -> complex out.
+template <bool BIG>
 __global__ __launch_bounds__(256) void k_line(LineArgs a) {
     extern __shared__ double2 lds[];
     if (a.skip && *a.skip) return;
@@ -162,7 +171,7 @@ __global__ __launch_bounds__(256) void k_line(LineArgs a) {
     int B = n;
     if (a.mode != LINE_IFFT) {
         for (int s = 0; s < a.nst; s++) {
-            run_stage<false>(a.radix[s], lds, pitch, T, n, B, a.tw);
+            run_stage<false, BIG>(a.radix[s], lds, pitch, T, n, B, a.tw);
             __syncthreads();
             B /= a.radix[s];
         }
@@ -185,7 +194,7 @@ __global__ __launch_bounds__(256) void k_line(LineArgs a) {
         }
         for (int s = a.nst - 1; s >= 0; s--) {
             B *= a.radix[s];
-            run_stage<true>(a.radix[s], lds, pitch, T, n, B, a.tw);
+            run_stage<true, BIG>(a.radix[s], lds, pitch, T, n, B, a.tw);
             __syncthreads();
         }
         for (int f = threadIdx.x; f < T * n; f += blockDim.x) {
@@ -245,7 +254,10 @@ hipError_t launch_line(const LineArgs& a, hipStream_t st) {
     }
     const long nblk = (a.nlines + a.T - 1) / a.T;
     const size_t lds = sizeof(double2) * (size_t)a.T * a.pitch;
-    hipLaunchKernelGGL(k_line, dim3((unsigned)nblk), dim3(256), lds, st, a);
+    bool big = false;
+    for (int s = 0; s < a.nst; s++) big = big || a.radix[s] > 7;
+    if (big) hipLaunchKernelGGL(k_line<true>, dim3((unsigned)nblk), dim3(256), lds, st, a);
+    else hipLaunchKernelGGL(k_line<false>, dim3((unsigned)nblk), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 

@@ -105,7 +105,7 @@ def test_create_rejects_bad_descriptors():
         L.Context(10, 5, (4, 4, 4), (1.0, -1.0, 1.0))
     with pytest.raises(L.LftsError):
         L.Context(10, 5, (4, 4, 4), (1.0, 1.0, 1.0), nhMax=0)
-    for m in ((22, 4, 4), (4, 13, 4), (4, 4, 0)):  # a prime factor above 7, an empty axis
+    for m in ((34, 4, 4), (4, 19, 4), (4, 4, 0)):  # a prime factor above 13, an empty axis
         with pytest.raises(L.LftsError) as e:
             L.Context(10, 5, m, (1.0, 1.0, 1.0))
         assert e.value.code == L.LFTS_EINVAL
@@ -116,6 +116,16 @@ def test_create_without_gpu_fails_loudly():
     with pytest.raises(L.LftsError) as e:
         L.Context(10, 5, (4, 4, 4), (1.0, 1.0, 1.0))
     assert e.value.code == L.LFTS_EHIP
+
+
+@pytest.mark.skipif(has_gpu(), reason="checks the no-device error path")
+def test_create_accepts_factors_11_and_13():
+    """Mesh sizes with factors 11 and 13 pass the size check (the generic line transform
+    has radix-11 and radix-13 stages): without a device they fail only there."""
+    for m in ((22, 4, 4), (4, 13, 4), (11, 26, 143)):
+        with pytest.raises(L.LftsError) as e:
+            L.Context(10, 5, m, (1.0, 1.0, 1.0))
+        assert e.value.code == L.LFTS_EHIP
 
 
 def test_driver_missing_input(tmp_path):

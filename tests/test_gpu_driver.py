@@ -23,13 +23,16 @@ def _numbers(path):
     return np.array([float(x) for x in open(path).read().split()])
 
 
-def test_driver_matches_oracle_outputs(gpu, tmp_path):
-    m, N, NA = (12, 12, 12), 16, 8
+# a cubic slab-path mesh, and a mesh with factors 11 and 13 (generic line kernel)
+@pytest.mark.parametrize("m,Lb", [((12, 12, 12), (3.0, 3.0, 3.0)), ((22, 13, 6), (3.6, 2.2, 1.1))])
+def test_driver_matches_oracle_outputs(gpu, tmp_path, m, Lb):
+    N, NA = 16, 8
     M = int(np.prod(m))
     rng = np.random.default_rng(21)
     w0 = np.concatenate([rng.normal(0, 0.1, M), rng.normal(0, 0.1, M)])
     inp = tmp_path / "input"
-    _write_input(inp, f"{N} {NA} 14.0 50 0.5 1\n12 12 12 3.0 3.0 3.0\n4 4 2 2 1\n", w0)
+    mesh = " ".join(str(x) for x in m) + " " + " ".join(str(x) for x in Lb)
+    _write_input(inp, f"{N} {NA} 14.0 50 0.5 1\n{mesh}\n4 4 2 2 1\n", w0)
     gdir, odir = tmp_path / "gpu", tmp_path / "orc"
     gdir.mkdir()
     odir.mkdir()

@@ -23,6 +23,8 @@ MESHES = [
     ((32, 32, 32), (4.0, 4.0, 4.0), 64, 32),
     ((32, 16, 24), (3.5, 2.8, 3.1), 10, 4),    # line path, register-FFT axis passes (edge lengths)
     ((64, 40, 48), (4.1, 3.3, 3.7), 8, 5),
+    ((22, 13, 8), (3.1, 2.2, 1.9), 10, 4),     # radices 11 and 13 (generic line kernel)
+    ((11, 26, 6), (2.0, 3.3, 1.5), 9, 5),
 ]
 
 
@@ -159,7 +161,7 @@ def test_langevin_step_host_and_device_noise(gpu):
 
 
 @pytest.mark.parametrize("m,Lb", [((8, 6, 10), (3.0, 2.5, 4.0)), ((40, 40, 40), (4.38, 4.38, 4.38)),
-                                  ((7, 9, 5), (2.0, 2.0, 2.0))])
+                                  ((7, 9, 5), (2.0, 2.0, 2.0)), ((11, 13, 4), (2.0, 2.3, 1.0))])
 def test_structure_function(gpu, m, Lb):
     M = int(np.prod(m))
     ctx = L.Context(10, 5, m, Lb)

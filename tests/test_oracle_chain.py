@@ -17,7 +17,7 @@ def brute_dft(x: np.ndarray, sign: int = -1) -> np.ndarray:
     return W @ x
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 15, 16, 20, 24, 40, 48, 64, 11, 13])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 15, 16, 20, 24, 40, 48, 64, 11, 13, 22, 26, 44, 143])
 def test_fft1d_against_brute_force(n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
@@ -27,7 +27,7 @@ def test_fft1d_against_brute_force(n):
         assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 4), (6, 5, 8), (3, 7, 2), (8, 1, 6), (5, 5, 5)])
+@pytest.mark.parametrize("shape", [(4, 4, 4), (6, 5, 8), (3, 7, 2), (8, 1, 6), (5, 5, 5), (11, 13, 2)])
 def test_fft3d_against_brute_force_and_numpy(shape):
     rng = np.random.default_rng(sum(shape))
     x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
