@@ -134,7 +134,7 @@ int main(int argc, char* argv[]) {
             d.L[k] = r.P->L()[k];
         }
         d.nhMax = 10;
-        if (lfts_create(&d, &r.ctx) != LFTS_OK) die("cannot create the GPU context of " + in);
+        if (const int rc = lfts_create(&d, &r.ctx)) die("cannot create the GPU context of " + in + ": " + create_error_text(rc));
         r.eta[0].assign(r.M, 0.0);
         r.eta[1].assign(r.M, 0.0);
         lfts_rng_normals(r.rng, r.eta[0].data(), r.M);  // src/langevin.cc:16

@@ -2,6 +2,8 @@
 #pragma once
 #include <string>
 
+#include "lfts_gpu.h"
+
 class lfts_params {
     int N_ = 0, NA_ = 0, NB_ = 0;
     double chi_b_ = 0, C_ = 0, dt_ = 0;
@@ -41,3 +43,12 @@ class lfts_params {
     double sigma() const { return sigma_; }
     double n() const { return C_ * V_; }  // number of polymers, C*V
 };
+
+// Why lfts_create refused a run's parameters, for the drivers' error line.
+inline std::string create_error_text(int rc) {
+    if (rc == LFTS_EINVAL)
+        return "the input cannot run on the GPU: mesh sizes need prime factors among 2, 3, 5, 7, 11 and 13, "
+               "N >= 2, 0 <= NA <= N and box edges > 0";
+    if (rc == LFTS_EHIP) return "no usable GPU (HIP error)";
+    return "error code " + std::to_string(rc);
+}

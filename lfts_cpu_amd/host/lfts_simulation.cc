@@ -46,7 +46,7 @@ lfts_simulation::lfts_simulation(const std::string& inputFile, const run_options
     d.nhMax = 10;
     int rc = lfts_create(&d, &ctx_);
     if (rc != LFTS_OK) {
-        std::cout << "ERROR => cannot create the GPU context (code " << rc << ")" << std::endl;
+        std::cout << "ERROR => cannot create the GPU context: " << create_error_text(rc) << std::endl;
         std::exit(1);
     }
     std::cout << "Creating anderson object..." << std::endl;

@@ -134,6 +134,16 @@ def test_driver_missing_input(tmp_path):
     assert "ERROR => Cannot open one of the input files." in r.stdout
 
 
+def test_driver_names_an_unsupported_mesh(tmp_path):
+    """A mesh size with a prime factor above 13 stops the driver with the reason (the size
+    check comes before any device call)."""
+    f = tmp_path / "in"
+    f.write_text("10 5 14.0 50 0.5 0\n17 4 4 3.0 3.0 3.0\n2 2 1 1 0\n")
+    r = subprocess.run([L.DRIVER_PATH, str(f)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 1
+    assert "mesh sizes need prime factors among 2, 3, 5, 7, 11 and 13" in r.stdout
+
+
 def test_driver_usage():
     r = subprocess.run([L.DRIVER_PATH], capture_output=True, text=True)
     assert r.returncode == 1 and "Please supply an input file name" in r.stdout
