@@ -44,9 +44,9 @@ typedef struct lfts_ctx lfts_ctx;
 /* Replaces the constructors diblock(NA,NB,m,L,M,Mk) (src/diblock.cc:9-25),
  * step(...) with its g(k) table (src/step.cc:10-28,69-89), anderson(M,10)
  * (src/anderson.cc:8-21) and the device state of langevin/strFunc. Returns LFTS_EINVAL
- * for N < 2, NA outside 0..N, nhMax outside 1..15, a box edge <= 0 or a mesh size with a
- * prime factor other than 2, 3, 5, 7, 11 and 13 (checked before any device call),
- * LFTS_EHIP without a device. */
+ * for N < 2, NA outside 0..N, nhMax outside 1..15, a box edge <= 0 or a mesh size of more
+ * than 2047 points with a prime factor above 7 (checked before any device call), LFTS_EHIP
+ * without a device. */
 int lfts_create(const lfts_desc* desc, lfts_ctx** out);
 int lfts_destroy(lfts_ctx* ctx);
 const char* lfts_last_error(const lfts_ctx* ctx);

@@ -50,13 +50,22 @@ int bfail(lfts_batch* b, int code, const std::string& msg) {
     return code;
 }
 
+// Stage radices of a line transform of n points: 4s, 2s, 3, 5 and 7, then any larger primes
+// (direct-DFT stages of the generic line kernel, for lines of up to lfts::MAX_PRIME_LINE
+// points, whose two LDS tiles then stay within 64 KB).
 bool factorize(int n, std::vector<int>& r) {
     r.clear();
+    if (n < 1) return false;
+    const int n0 = n;
     while (n % 4 == 0) { r.push_back(4); n /= 4; }
     while (n % 2 == 0) { r.push_back(2); n /= 2; }
-    for (int p : {3, 5, 7, 11, 13})
+    for (int p : {3, 5, 7})
         while (n % p == 0) { r.push_back(p); n /= p; }
-    return n == 1;
+    if (n == 1) return true;
+    for (int p = 11; p * p <= n; p += 2)
+        while (n % p == 0) { r.push_back(p); n /= p; }
+    if (n > 1) r.push_back(n);
+    return n0 <= lfts::MAX_PRIME_LINE;
 }
 
 // Position of frequency k after the in-place decimation-in-frequency pass: a stage of
@@ -71,7 +80,7 @@ int dif_pos(int k, const std::vector<int>& rad, size_t s, int B) {
 int build_axis(lfts_ctx* c, AxisPlan& p, int n, double L, int N) {
     std::vector<int> rad;
     if (!factorize(n, rad) || (int)rad.size() > lfts::MAXST)
-        return fail(c, LFTS_EINVAL, "mesh size " + std::to_string(n) + " has a prime factor above 7");
+        return fail(c, LFTS_EINVAL, "mesh size " + std::to_string(n) + " is not supported");
     p.n = n;
     p.nst = (int)rad.size();
     for (int s = 0; s < p.nst; s++) p.radix[s] = rad[s];
@@ -103,8 +112,11 @@ int build_axis(lfts_ctx* c, AxisPlan& p, int n, double L, int N) {
     HIPCHK(c, hipMemcpy(p.gperm, gperm.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(p.perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(p.gnat, gnat.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    p.T = std::max(1, std::min(16, 4096 / n));
     p.pitch = n + 1;
+    p.T = std::max(1, std::min(16, 4096 / n));
+    bool prime = false;  // a stage of radix 11 or more: the kernel holds two tiles of T lines
+    for (int s = 0; s < p.nst; s++) prime = prime || p.radix[s] > 7;
+    if (prime) p.T = std::max(1, std::min(16, 2048 / p.pitch));
     return LFTS_OK;
 }
 
@@ -1127,7 +1139,7 @@ int lfts_create(const lfts_desc* desc, lfts_ctx** out) {
     if (desc->nhMax < 1 || desc->nhMax > lfts::MAX_HIST) return LFTS_EINVAL;
     for (int d = 0; d < 3; d++) {
         if (desc->m[d] < 1 || !(desc->L[d] > 0.0)) return LFTS_EINVAL;
-        std::vector<int> rad;  // the line transforms take factors 2, 3, 5, 7, 11 and 13 only
+        std::vector<int> rad;  // any factors; a prime factor above 7 up to MAX_PRIME_LINE points
         if (!factorize(desc->m[d], rad) || (int)rad.size() > lfts::MAXST) return LFTS_EINVAL;
     }
     int ndev = 0;

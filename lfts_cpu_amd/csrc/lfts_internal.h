@@ -10,6 +10,7 @@
 namespace lfts {
 
 constexpr int MAXST = 12;     // max FFT stages per axis (generic line path)
+constexpr int MAX_PRIME_LINE = 2047;  // longest axis with a prime factor above 7 (two LDS tiles <= 64 KB)
 constexpr int MAX_HIST = 15;  // max Anderson history depth supported on the device
 constexpr int PSTRIDE = 16;   // doubles per workgroup in the reduction partials buffer
 constexpr int SSTRIDE = 32;   // doubles of scalar results per replica

@@ -107,22 +107,57 @@ __device__ void line_stage(double2* lds, int pitch, int T, int n, int B, const d
     }
 }
 
-// BIG: also the radix-11 and radix-13 stages (direct DFTs), in a kernel of their own, since
-// their registers would halve the occupancy of the kernel that serves the other lengths
-template <bool INV, bool BIG>
+// A stage of any prime radix R (direct DFT, R products per output), out of place: each
+// thread forms single outputs of the butterflies into tmp (the line tile's layout), then
+// copies them back. The same operations as line_stage: forward, the butterfly then the
+// twiddle w_B^{j s}; inverse, the conjugate twiddle then the inverse butterfly.
+template <bool INV>
+__device__ void prime_stage(int R, double2* lds, double2* tmp, int pitch, int T, int n, int B,
+                            const double2* __restrict__ tw) {
+    const int S = B / R, tstep = n / B, st = n / R;
+    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+        const int t = idx / n, e = idx - t * n;
+        const int blk = e / B, rem = e - blk * B;
+        const int s = rem / S, j = rem - s * S;  // output s of butterfly j of block blk
+        const double2* p = lds + t * pitch + blk * B + j;
+        double2 acc = p[0];
+        for (int q = 1; q < R; q++) {
+            double2 v = p[q * S];
+            const double2 w = tw[((q * s) % R) * st];
+            if constexpr (INV) {
+                v = cmulc(v, tw[j * q * tstep]);
+                acc = cadd(acc, cmulc(v, w));
+            } else {
+                acc = cadd(acc, cmul(v, w));
+            }
+        }
+        if (!INV && s > 0) acc = cmul(acc, tw[j * s * tstep]);
+        tmp[t * pitch + e] = acc;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < T * n; idx += blockDim.x) {
+        const int t = idx / n, e = idx - t * n;
+        lds[t * pitch + e] = tmp[t * pitch + e];
+    }
+}
+
+// Radices 2, 3, 4, 5 and 7 unrolled; with PRIME, the primes from 11 up as prime_stage, in a
+// kernel of its own whose second tile (tmp) doubles its LDS (unrolled radix-11 and radix-13
+// stages took 251 VGPRs, prime_stage keeps the kernel at four waves per SIMD).
+template <bool INV, bool PRIME>
 __device__ void run_stage(int r, double2* lds, int pitch, int T, int n, int B, const double2* tw) {
+    if constexpr (PRIME) {
+        if (r >= 11) {
+            prime_stage<INV>(r, lds, lds + T * pitch, pitch, T, n, B, tw);
+            return;
+        }
+    }
     switch (r) {
         case 2: line_stage<2, INV>(lds, pitch, T, n, B, tw); break;
         case 3: line_stage<3, INV>(lds, pitch, T, n, B, tw); break;
         case 4: line_stage<4, INV>(lds, pitch, T, n, B, tw); break;
         case 5: line_stage<5, INV>(lds, pitch, T, n, B, tw); break;
         case 7: line_stage<7, INV>(lds, pitch, T, n, B, tw); break;
-        case 11:
-            if constexpr (BIG) line_stage<11, INV>(lds, pitch, T, n, B, tw);
-            break;
-        case 13:
-            if constexpr (BIG) line_stage<13, INV>(lds, pitch, T, n, B, tw);
-            break;
         default: break;  // plans only contain the radices above (checked on the host)
     }
 }
@@ -144,7 +179,7 @@ __device__ __forceinline__ long line_base(const LineArgs& a, long line) {
 //  mode LINE_IFFT: natural-order complex in -> inverse FFT -> complex out, or its real
 //                  part (outR).
 //  mode LINE_CMUL: complex in -> FFT -> *mul (per point) -> IFFT -> complex out.
-template <bool BIG>
+template <bool PRIME>
 __global__ __launch_bounds__(256) void k_line(LineArgs a) {
     extern __shared__ double2 lds[];
     if (a.skip && *a.skip) return;
@@ -171,7 +206,7 @@ __global__ __launch_bounds__(256) void k_line(LineArgs a) {
     int B = n;
     if (a.mode != LINE_IFFT) {
         for (int s = 0; s < a.nst; s++) {
-            run_stage<false, BIG>(a.radix[s], lds, pitch, T, n, B, a.tw);
+            run_stage<false, PRIME>(a.radix[s], lds, pitch, T, n, B, a.tw);
             __syncthreads();
             B /= a.radix[s];
         }
@@ -194,7 +229,7 @@ __global__ __launch_bounds__(256) void k_line(LineArgs a) {
         }
         for (int s = a.nst - 1; s >= 0; s--) {
             B *= a.radix[s];
-            run_stage<true, BIG>(a.radix[s], lds, pitch, T, n, B, a.tw);
+            run_stage<true, PRIME>(a.radix[s], lds, pitch, T, n, B, a.tw);
             __syncthreads();
         }
         for (int f = threadIdx.x; f < T * n; f += blockDim.x) {
@@ -254,9 +289,9 @@ hipError_t launch_line(const LineArgs& a, hipStream_t st) {
     }
     const long nblk = (a.nlines + a.T - 1) / a.T;
     const size_t lds = sizeof(double2) * (size_t)a.T * a.pitch;
-    bool big = false;
-    for (int s = 0; s < a.nst; s++) big = big || a.radix[s] > 7;
-    if (big) hipLaunchKernelGGL(k_line<true>, dim3((unsigned)nblk), dim3(256), lds, st, a);
+    bool prime = false;  // a stage of radix 11 or more
+    for (int s = 0; s < a.nst; s++) prime = prime || a.radix[s] > 7;
+    if (prime) hipLaunchKernelGGL(k_line<true>, dim3((unsigned)nblk), dim3(256), 2 * lds, st, a);
     else hipLaunchKernelGGL(k_line<false>, dim3((unsigned)nblk), dim3(256), lds, st, a);
     return hipGetLastError();
 }

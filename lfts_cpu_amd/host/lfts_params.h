@@ -47,8 +47,8 @@ class lfts_params {
 // Why lfts_create refused a run's parameters, for the drivers' error line.
 inline std::string create_error_text(int rc) {
     if (rc == LFTS_EINVAL)
-        return "the input cannot run on the GPU: mesh sizes need prime factors among 2, 3, 5, 7, 11 and 13, "
-               "N >= 2, 0 <= NA <= N and box edges > 0";
+        return "the input cannot run on the GPU: a mesh size with a prime factor above 7 may have at most 2047 "
+               "points, N >= 2, 0 <= NA <= N and box edges > 0";
     if (rc == LFTS_EHIP) return "no usable GPU (HIP error)";
     return "error code " + std::to_string(rc);
 }

@@ -105,7 +105,7 @@ def test_create_rejects_bad_descriptors():
         L.Context(10, 5, (4, 4, 4), (1.0, -1.0, 1.0))
     with pytest.raises(L.LftsError):
         L.Context(10, 5, (4, 4, 4), (1.0, 1.0, 1.0), nhMax=0)
-    for m in ((34, 4, 4), (4, 19, 4), (4, 4, 0)):  # a prime factor above 13, an empty axis
+    for m in ((2053, 4, 4), (4, 11 * 256, 4), (4, 4, 0)):  # a prime above 7 on a long axis, an empty axis
         with pytest.raises(L.LftsError) as e:
             L.Context(10, 5, m, (1.0, 1.0, 1.0))
         assert e.value.code == L.LFTS_EINVAL
@@ -119,10 +119,11 @@ def test_create_without_gpu_fails_loudly():
 
 
 @pytest.mark.skipif(has_gpu(), reason="checks the no-device error path")
-def test_create_accepts_factors_11_and_13():
-    """Mesh sizes with factors 11 and 13 pass the size check (the generic line transform
-    has radix-11 and radix-13 stages): without a device they fail only there."""
-    for m in ((22, 4, 4), (4, 13, 4), (11, 26, 143)):
+def test_create_accepts_any_prime_factor():
+    """Mesh sizes with prime factors above 7 (11, 13 and larger) on axes of up to 2047
+    points pass the size check (the generic line transform has direct-DFT stages of any
+    prime): without a device they fail only there."""
+    for m in ((22, 4, 4), (4, 13, 4), (11, 26, 143), (17, 4, 4), (4, 38, 23), (2047, 1, 2)):
         with pytest.raises(L.LftsError) as e:
             L.Context(10, 5, m, (1.0, 1.0, 1.0))
         assert e.value.code == L.LFTS_EHIP
@@ -135,13 +136,13 @@ def test_driver_missing_input(tmp_path):
 
 
 def test_driver_names_an_unsupported_mesh(tmp_path):
-    """A mesh size with a prime factor above 13 stops the driver with the reason (the size
-    check comes before any device call)."""
+    """A mesh size with a prime factor above 7 on an axis of more than 2047 points stops
+    the driver with the reason (the size check comes before any device call)."""
     f = tmp_path / "in"
-    f.write_text("10 5 14.0 50 0.5 0\n17 4 4 3.0 3.0 3.0\n2 2 1 1 0\n")
+    f.write_text("10 5 14.0 50 0.5 0\n2053 2 2 3.0 3.0 3.0\n2 2 1 1 0\n")
     r = subprocess.run([L.DRIVER_PATH, str(f)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert r.returncode == 1
-    assert "mesh sizes need prime factors among 2, 3, 5, 7, 11 and 13" in r.stdout
+    assert "a mesh size with a prime factor above 7 may have at most 2047 points" in r.stdout
 
 
 def test_driver_usage():

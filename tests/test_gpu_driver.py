@@ -23,8 +23,10 @@ def _numbers(path):
     return np.array([float(x) for x in open(path).read().split()])
 
 
-# a cubic slab-path mesh, and a mesh with factors 11 and 13 (generic line kernel)
-@pytest.mark.parametrize("m,Lb", [((12, 12, 12), (3.0, 3.0, 3.0)), ((22, 13, 6), (3.6, 2.2, 1.1))])
+# a cubic slab-path mesh, and meshes with factors 11 and 13 and with a larger prime (generic
+# line kernel)
+@pytest.mark.parametrize("m,Lb", [((12, 12, 12), (3.0, 3.0, 3.0)), ((22, 13, 6), (3.6, 2.2, 1.1)),
+                                  ((17, 8, 6), (2.8, 1.4, 1.1))])
 def test_driver_matches_oracle_outputs(gpu, tmp_path, m, Lb):
     N, NA = 16, 8
     M = int(np.prod(m))

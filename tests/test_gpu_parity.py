@@ -25,6 +25,8 @@ MESHES = [
     ((64, 40, 48), (4.1, 3.3, 3.7), 8, 5),
     ((22, 13, 8), (3.1, 2.2, 1.9), 10, 4),     # radices 11 and 13 (generic line kernel)
     ((11, 26, 6), (2.0, 3.3, 1.5), 9, 5),
+    ((17, 6, 38), (2.6, 1.8, 3.4), 8, 3),      # direct-DFT stages of primes 17 and 19
+    ((23, 4, 5), (2.7, 1.2, 1.5), 7, 4),
 ]
 
 
@@ -161,7 +163,8 @@ def test_langevin_step_host_and_device_noise(gpu):
 
 
 @pytest.mark.parametrize("m,Lb", [((8, 6, 10), (3.0, 2.5, 4.0)), ((40, 40, 40), (4.38, 4.38, 4.38)),
-                                  ((7, 9, 5), (2.0, 2.0, 2.0)), ((11, 13, 4), (2.0, 2.3, 1.0))])
+                                  ((7, 9, 5), (2.0, 2.0, 2.0)), ((11, 13, 4), (2.0, 2.3, 1.0)),
+                                  ((17, 19, 2), (2.2, 2.4, 0.8))])
 def test_structure_function(gpu, m, Lb):
     M = int(np.prod(m))
     ctx = L.Context(10, 5, m, Lb)

@@ -17,7 +17,7 @@ def brute_dft(x: np.ndarray, sign: int = -1) -> np.ndarray:
     return W @ x
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 15, 16, 20, 24, 40, 48, 64, 11, 13, 22, 26, 44, 143])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 15, 16, 20, 24, 40, 48, 64, 11, 13, 22, 26, 44, 143, 17, 34, 323])
 def test_fft1d_against_brute_force(n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
