@@ -27,6 +27,7 @@ MESHES = [
     ((11, 26, 6), (2.0, 3.3, 1.5), 9, 5),
     ((17, 6, 38), (2.6, 1.8, 3.4), 8, 3),      # direct-DFT stages of primes 17 and 19
     ((23, 4, 5), (2.7, 1.2, 1.5), 7, 4),
+    ((2, 131, 3), (0.9, 6.5, 1.2), 6, 2),      # a long prime axis: 131 products per output
 ]
 
 
