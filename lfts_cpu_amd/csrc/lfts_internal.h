@@ -62,7 +62,7 @@ struct Coef {
     int keep_new;
 };
 
-// ---- generic line kernels (any mesh with factors 2, 3, 5, 7), one replica ----------
+// ---- generic line kernels (any mesh; see factorize), one replica ----------------------
 struct LineArgs {
     int n, nst, radix[MAXST];
     const double2* tw;    // n entries, exp(-2 pi i k/n)
