@@ -271,3 +271,29 @@ def test_host_mirrored_scalars_equal_copied_ones_for_one_replica(gpu, monkeypatc
         assert r == out[0][0]
         assert np.array_equal(f, out[0][1])
         assert n == out[0][2]
+
+
+# a member whose axes have prime factors 13 and 17 (direct-DFT stages of the line kernel)
+PRIME_MEMBERS = [MEMBERS[0], ((13, 17, 4), (2.6, 3.1, 1.0), 10, 4, 6), MEMBERS[3]]
+
+
+def test_prime_mesh_member_in_set_and_graphs(gpu, monkeypatch):
+    """A member of prime-factor mesh sizes gives in a set the bits it gets alone, and the
+    set's replayed iteration graphs (which then hold that kernel's launches) give the bits
+    of the direct launches."""
+    ctxs, _ = _make(PRIME_MEMBERS)
+    b = L.Batch(ctxs)
+    lnq = b.calc_concs()
+    got = [c.get_fields(4) for c in ctxs]
+    b.close()
+    alone, _ = _make(PRIME_MEMBERS)
+    for i, c in enumerate(alone):
+        assert c.calc_concs() == lnq[i]
+        assert np.array_equal(c.get_fields(4), got[i]), i
+    g = _steps(PRIME_MEMBERS, monkeypatch, True)
+    d = _steps(PRIME_MEMBERS, monkeypatch, False)
+    assert np.array_equal(g[0], d[0])
+    for (lg, kg), (ld, kd) in zip(g[1], d[1]):
+        assert np.array_equal(lg, ld) and np.array_equal(kg, kd)
+    for fg, fd in zip(g[2], d[2]):
+        assert np.array_equal(fg, fd)
