@@ -166,7 +166,6 @@ __global__ __launch_bounds__(AX_LINES * R2) void k_axis(const LineArgs a) {
     constexpr int XP = ZAX ? zline_pitch(R1, R2, (R1 * (R2 + 1)) | 1) : (R1 * (R2 + 1)) | 1;
     constexpr int XBYTES = AX_LINES * XP * 16;
     __shared__ __align__(16) unsigned char smem[XBYTES + E * 16 + E * 8];
-    if (a.skip && *a.skip) return;
     const int t = threadIdx.x;
     const int c = ZAX ? t / R2 : t % AX_LINES;  // line within the workgroup
     const int j = ZAX ? t % R2 : t / AX_LINES;  // lane within the line
@@ -180,6 +179,9 @@ __global__ __launch_bounds__(AX_LINES * R2) void k_axis(const LineArgs a) {
         const long p = base + (long)(j + R2 * k) * S;
         z[k] = live ? make_double2(a.inA[p], a.inB[p]) : make_double2(0.0, 0.0);
     }
+    // the skip flag once the line loads are out (they read only the pass's own operands):
+    // tested first, its argument load and wait would come before the others
+    if (a.skip && *a.skip) return;
     double2* stw = reinterpret_cast<double2*>(smem + XBYTES);
     double* sg = reinterpret_cast<double*>(smem + XBYTES + E * 16);
     for (int k = t; k < E; k += AX_LINES * R2) {

@@ -182,7 +182,6 @@ __device__ __forceinline__ long line_base(const LineArgs& a, long line) {
 template <bool PRIME>
 __global__ __launch_bounds__(256) void k_line(LineArgs a) {
     extern __shared__ double2 lds[];
-    if (a.skip && *a.skip) return;
     const int n = a.n, T = a.T, pitch = a.pitch;
     const long line0 = (long)blockIdx.x * T;
     const bool zaxis = (a.axis == 2);
@@ -201,6 +200,9 @@ __global__ __launch_bounds__(256) void k_line(LineArgs a) {
         // the inverse-only mode places frequency e where the forward pass would leave it
         lds[t * pitch + (a.mode == LINE_IFFT ? a.perm[e] : e)] = v;
     }
+    // the skip flag after the line loads (tested first, its argument load and wait would
+    // come before the others); the whole workgroup returns before any barrier
+    if (a.skip && *a.skip) return;
     __syncthreads();
 
     int B = n;
